@@ -1,0 +1,136 @@
+/*
+ * sbdx.h -- C ABI of the MI355X selected-basis diagonalization engine (libsbdx.so).
+ *
+ * One context holds one selected-basis problem on one GPU: the alpha and beta string tables,
+ * their single/double excitation lists, and the integrals.  Every entry point returns 0 on
+ * success and a negative sbdx_status on failure (message: sbdx_last_error()); no C++
+ * exception crosses this boundary.  Host buffers are owned by the caller, device buffers by
+ * the context.  Calls on one context must not overlap (one context per host thread/stream).
+ *
+ * Vectors are determinant-indexed, alpha-major: W[ia * n_beta + ib] (the reference's layout,
+ * chemistry/tpb/mult.h:503-504, qcham.h:359).  Strings are uint64 bit masks, bit p = orbital
+ * p, i.e. the reference's `bit_length`-bit words concatenated (framework/bit_manipulation.h:
+ * 863-893 change_bitlength to 64); norb <= 64.
+ *
+ * Entry points and the reference interface each one stands in for (file:line in r-ccs-cms/sbd
+ * include/sbd/):
+ *   sbdx_create            MakeHelpers / RemakeHelpers    chemistry/tpb/helper.h:432, :650
+ *   sbdx_set_integrals     SetupIntegrals (consumer side) chemistry/basic/makeintegrals.h:10
+ *   sbdx_get_excitations   TaskHelpers lists              chemistry/tpb/helper.h:12-36, :132-182
+ *   sbdx_hdiag             makeQChamDiagTerms             chemistry/tpb/qcham.h:314-366
+ *   sbdx_sigma             mult (on-the-fly)              chemistry/tpb/mult.h:387-648
+ *   sbdx_davidson          Davidson (+ max_time)          chemistry/tpb/davidson.h:661-923
+ *   sbdx_rdm               Correlation                    chemistry/tpb/correlation.h:194-443
+ *   sbdx_occupation        OccupationDensity              chemistry/tpb/occupation.h:10-95
+ *   sbdx_destroy           FreeHelpers                    chemistry/tpb/helper.h:339-350
+ */
+#ifndef SBDX_H
+#define SBDX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  SBDX_OK = 0,
+  SBDX_ERR_ARG = -1,       /* invalid argument (sizes, null pointers, norb > 64, ...) */
+  SBDX_ERR_STATE = -2,     /* call out of order (e.g. sigma before set_integrals) */
+  SBDX_ERR_DEVICE = -3,    /* HIP runtime error (allocation, launch) */
+  SBDX_ERR_NOCONV = -4     /* not an error of the call: Davidson stopped before tolerance */
+} sbdx_status;
+
+/* Arithmetic of a sigma build. */
+enum { SBDX_FP64 = 0, SBDX_FP32 = 1 };
+
+typedef struct sbdx_ctx sbdx_ctx;
+
+/* Create a context on the current HIP device.  alpha/beta: n_alpha/n_beta uint64 strings (any
+ * order; each string's index is its row/column in W).  Builds the excitation lists on the GPU. */
+int sbdx_create(int norb, int64_t n_alpha, const uint64_t *alpha, int64_t n_beta,
+                const uint64_t *beta, sbdx_ctx **out);
+
+/* Integrals in the reference's storage: I1 = oneInt::store, (2 norb)^2 spin-orbital entries,
+ * I1[i*2L+j] (even index = alpha, odd = beta); I2 = twoInt::store, packed spatial (pq|rs) of
+ * n_I2 = B(B+1)/2 entries with B = norb(norb+1)/2 (chemistry/basic/integrals.h:10-67). */
+int sbdx_set_integrals(sbdx_ctx *ctx, double I0, const double *I1, const double *I2,
+                       int64_t n_I2);
+
+/* Sizes of the excitation lists of one spin (0 = alpha, 1 = beta). */
+int sbdx_excitation_counts(sbdx_ctx *ctx, int spin, int64_t *n_single, int64_t *n_double);
+
+/* Copy one spin's lists to host CSR: off_* have n+1 entries; idx_* hold ket string indices in
+ * ascending order per bra string (the reference's SinglesFrom* / DoublesFrom* lists). */
+int sbdx_get_excitations(sbdx_ctx *ctx, int spin, int64_t *off_s, int64_t *idx_s,
+                         int64_t *off_d, int64_t *idx_d);
+
+int64_t sbdx_ndet(const sbdx_ctx *ctx);
+
+/* H_ii for every determinant (host buffer of sbdx_ndet entries). */
+int sbdx_hdiag(sbdx_ctx *ctx, double *hii);
+
+/* S = H C for nvec vectors (host buffers, nvec * ndet doubles each, vector-major).  Unlike
+ * the reference's mult the output is overwritten, not accumulated. */
+int sbdx_sigma(sbdx_ctx *ctx, int nvec, const double *C, double *S);
+
+/* Device-resident variant: dC, dS are device pointers of nvec*ndet elements of the type given
+ * by prec (SBDX_FP64: double, SBDX_FP32: float); stream may be NULL (context stream). */
+int sbdx_sigma_device(sbdx_ctx *ctx, int prec, int nvec, const void *dC, void *dS,
+                      void *stream);
+
+/* Davidson options (defaults by sbdx_davidson_defaults = the reference's SBD struct,
+ * chemistry/tpb/sbdiag.h:12-30, with the iteration cap raised to converge). */
+typedef struct {
+  int max_iteration;      /* restarts (reference: max_it) */
+  int num_block;          /* subspace size per restart (reference: max_nb) */
+  double eps;             /* residual tolerance ||E0 W - H W|| */
+  double max_time;        /* wall-clock budget in seconds */
+  int init;               /* 0: W[0] = 1 (HF, reference init 0); 1: use W as given */
+  int precision;          /* SBDX_FP64 or SBDX_FP32 (mixed: FP32 sigma, FP64 refinement) */
+  int nroots;             /* lowest roots solved together (block Davidson), >= 1 */
+  int preconditioner;     /* 0: diagonal (Jacobi); 1: learned (falls back to 0 on stall) */
+  int verbose;            /* print the reference's per-step line */
+  double energy_ref;      /* if nonzero: record time to |E - energy_ref| < energy_tol */
+  double energy_tol;
+} sbdx_davidson_opts;
+
+typedef struct {
+  double energy;          /* <W|H|W> of the returned vector (reference: sbdiag.h:239-262) */
+  double residual;        /* last ||r|| */
+  int steps;              /* sigma builds inside the iteration */
+  int converged;
+  double seconds;         /* wall time of the iteration */
+  double seconds_to_tol;  /* time at which |E0 - energy_ref| < energy_tol, or -1 */
+  int steps_to_tol;
+} sbdx_davidson_result;
+
+void sbdx_davidson_defaults(sbdx_davidson_opts *opts);
+
+/* W: host buffer of nroots*ndet doubles (in: initial guess if init=1; out: Ritz vectors).
+ * trace/resid (optional, may be NULL): lowest Ritz value and residual norm per step. */
+int sbdx_davidson(sbdx_ctx *ctx, const sbdx_davidson_opts *opts, double *W,
+                  sbdx_davidson_result *res, double *trace, double *resid, int trace_cap);
+
+/* Device-resident variant: dW is a device buffer of nroots*ndet doubles. */
+int sbdx_davidson_device(sbdx_ctx *ctx, const sbdx_davidson_opts *opts, void *dW,
+                         sbdx_davidson_result *res, double *trace, double *resid,
+                         int trace_cap);
+
+/* 1- and 2-RDM of W (host): onebody[s][i + L*j] (2 L^2), twobody[s + 2t][i + L j + L^2 k + L^3 l]
+ * (4 L^4) = <c+_{is} c+_{jt} c_{lt} c_{ks}> (the reference's layout, user-guide.md:136-139). */
+int sbdx_rdm(sbdx_ctx *ctx, const double *W, double *onebody, double *twobody);
+
+/* Occupations density[2p + s] = <n_{p,s}> of W (host). */
+int sbdx_occupation(sbdx_ctx *ctx, const double *W, double *density);
+
+void sbdx_destroy(sbdx_ctx *ctx);
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char *sbdx_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

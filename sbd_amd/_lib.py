@@ -1,0 +1,225 @@
+"""ctypes binding of libsbdx.so (the C ABI declared in include/sbdx.h).
+
+The library is built in-tree (``python -m sbd_amd.build``); importing this module never
+falls back to any other implementation: a missing library is an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsbdx.so")
+
+SBDX_FP64 = 0
+SBDX_FP32 = 1
+
+EXPORTS = [
+    "sbdx_create", "sbdx_set_integrals", "sbdx_excitation_counts", "sbdx_get_excitations",
+    "sbdx_ndet", "sbdx_hdiag", "sbdx_sigma", "sbdx_sigma_device", "sbdx_davidson_defaults",
+    "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_destroy",
+    "sbdx_last_error",
+]
+
+
+class DavidsonOpts(C.Structure):
+    _fields_ = [("max_iteration", C.c_int), ("num_block", C.c_int), ("eps", C.c_double),
+                ("max_time", C.c_double), ("init", C.c_int), ("precision", C.c_int),
+                ("nroots", C.c_int), ("preconditioner", C.c_int), ("verbose", C.c_int),
+                ("energy_ref", C.c_double), ("energy_tol", C.c_double)]
+
+
+class DavidsonResult(C.Structure):
+    _fields_ = [("energy", C.c_double), ("residual", C.c_double), ("steps", C.c_int),
+                ("converged", C.c_int), ("seconds", C.c_double), ("seconds_to_tol", C.c_double),
+                ("steps_to_tol", C.c_int)]
+
+
+class SbdxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"sbdx error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -m sbd_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.sbdx_create.argtypes = [C.c_int, C.c_int64, u64p, C.c_int64, u64p, C.POINTER(vp)]
+    L.sbdx_set_integrals.argtypes = [vp, C.c_double, f64p, f64p, C.c_int64]
+    L.sbdx_excitation_counts.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.sbdx_get_excitations.argtypes = [vp, C.c_int, i64p, i64p, i64p, i64p]
+    L.sbdx_ndet.argtypes = [vp]
+    L.sbdx_ndet.restype = C.c_int64
+    L.sbdx_hdiag.argtypes = [vp, f64p]
+    L.sbdx_sigma.argtypes = [vp, C.c_int, f64p, f64p]
+    L.sbdx_sigma_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    L.sbdx_davidson_defaults.argtypes = [C.POINTER(DavidsonOpts)]
+    L.sbdx_davidson_defaults.restype = None
+    L.sbdx_davidson.argtypes = [vp, C.POINTER(DavidsonOpts), f64p, C.POINTER(DavidsonResult),
+                                vp, vp, C.c_int]
+    L.sbdx_davidson_device.argtypes = [vp, C.POINTER(DavidsonOpts), vp, C.POINTER(DavidsonResult),
+                                       vp, vp, C.c_int]
+    L.sbdx_rdm.argtypes = [vp, f64p, f64p, f64p]
+    L.sbdx_occupation.argtypes = [vp, f64p, f64p]
+    L.sbdx_destroy.argtypes = [vp]
+    L.sbdx_destroy.restype = None
+    L.sbdx_last_error.restype = C.c_char_p
+    for name in EXPORTS:
+        if name not in ("sbdx_ndet", "sbdx_davidson_defaults", "sbdx_destroy", "sbdx_last_error"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().sbdx_last_error().decode(errors="replace")
+        raise SbdxError(rc, msg)
+    return rc
+
+
+def davidson_defaults(**kw) -> DavidsonOpts:
+    o = DavidsonOpts()
+    lib().sbdx_davidson_defaults(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+class Engine:
+    """One selected-basis problem resident on the current GPU (an ``sbdx_ctx``)."""
+
+    def __init__(self, norb, alpha, beta, I0=None, I1=None, I2=None):
+        self.norb = int(norb)
+        self.alpha = np.ascontiguousarray(alpha, dtype=np.uint64)
+        self.beta = np.ascontiguousarray(beta, dtype=np.uint64)
+        h = C.c_void_p()
+        check(lib().sbdx_create(self.norb, self.alpha.size, self.alpha, self.beta.size, self.beta,
+                                C.byref(h)))
+        self._h = h
+        if I1 is not None:
+            self.set_integrals(I0, I1, I2)
+
+    @classmethod
+    def from_instance(cls, inst):
+        """Build from a synthetic Instance, with the integrals in the reference's storage."""
+        L = inst.norb
+        I1 = np.zeros((2 * L, 2 * L))
+        h = inst.ints.h1
+        I1[0::2, 0::2] = h
+        I1[1::2, 1::2] = h
+        return cls(L, inst.alpha, inst.beta, inst.ints.ecore, I1.ravel(), inst.ints.packed_eri())
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_integrals(self, I0, I1, I2):
+        I1 = np.ascontiguousarray(I1, dtype=np.float64).ravel()
+        I2 = np.ascontiguousarray(I2, dtype=np.float64).ravel()
+        check(lib().sbdx_set_integrals(self._h, float(I0), I1, I2, I2.size))
+
+    @property
+    def ndet(self):
+        return int(lib().sbdx_ndet(self._h))
+
+    def excitation_counts(self, spin):
+        ns, nd = C.c_int64(), C.c_int64()
+        check(lib().sbdx_excitation_counts(self._h, spin, C.byref(ns), C.byref(nd)))
+        return ns.value, nd.value
+
+    def excitations(self, spin):
+        n = self.alpha.size if spin == 0 else self.beta.size
+        ns, nd = self.excitation_counts(spin)
+        off_s = np.zeros(n + 1, np.int64)
+        off_d = np.zeros(n + 1, np.int64)
+        idx_s = np.zeros(max(ns, 1), np.int64)
+        idx_d = np.zeros(max(nd, 1), np.int64)
+        check(lib().sbdx_get_excitations(self._h, spin, off_s, idx_s, off_d, idx_d))
+        return off_s, idx_s[:ns], off_d, idx_d[:nd]
+
+    def hdiag(self):
+        out = np.zeros(self.ndet)
+        check(lib().sbdx_hdiag(self._h, out))
+        return out
+
+    def sigma(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        nvec = x.size // self.ndet
+        y = np.zeros_like(x)
+        check(lib().sbdx_sigma(self._h, nvec, x, y))
+        return y
+
+    def sigma_device(self, prec, nvec, dC, dS, stream=None):
+        check(lib().sbdx_sigma_device(self._h, prec, nvec, C.c_void_p(dC), C.c_void_p(dS),
+                                      C.c_void_p(stream) if stream else None))
+
+    def davidson(self, W=None, trace_cap=4096, **opts):
+        o = davidson_defaults(**opts)
+        if W is None:
+            W = np.zeros(self.ndet * max(1, o.nroots))
+            o.init = 0
+        else:
+            W = np.ascontiguousarray(W, dtype=np.float64).copy()
+            o.init = 1
+        res = DavidsonResult()
+        trace = np.zeros(trace_cap)
+        resid = np.zeros(trace_cap)
+        check(lib().sbdx_davidson(self._h, C.byref(o), W, C.byref(res), trace.ctypes.data,
+                                  resid.ctypes.data, trace_cap))
+        n = min(res.steps, trace_cap)
+        return dict(W=W, energy=res.energy, residual=res.residual, steps=res.steps,
+                    converged=bool(res.converged), seconds=res.seconds,
+                    seconds_to_tol=res.seconds_to_tol, steps_to_tol=res.steps_to_tol,
+                    trace=trace[:n].copy(), resid=resid[:n].copy())
+
+    def davidson_device(self, dW, trace_cap=4096, **opts):
+        o = davidson_defaults(**opts)
+        res = DavidsonResult()
+        trace = np.zeros(trace_cap)
+        resid = np.zeros(trace_cap)
+        check(lib().sbdx_davidson_device(self._h, C.byref(o), C.c_void_p(dW), C.byref(res),
+                                         trace.ctypes.data, resid.ctypes.data, trace_cap))
+        n = min(res.steps, trace_cap)
+        return dict(energy=res.energy, residual=res.residual, steps=res.steps,
+                    converged=bool(res.converged), seconds=res.seconds,
+                    seconds_to_tol=res.seconds_to_tol, steps_to_tol=res.steps_to_tol,
+                    trace=trace[:n].copy(), resid=resid[:n].copy())
+
+    def rdm(self, W):
+        L = self.norb
+        one = np.zeros(2 * L * L)
+        two = np.zeros(4 * L ** 4)
+        check(lib().sbdx_rdm(self._h, np.ascontiguousarray(W, np.float64), one, two))
+        return one.reshape(2, L * L), two.reshape(4, L ** 4)
+
+    def occupation(self, W):
+        d = np.zeros(2 * self.norb)
+        check(lib().sbdx_occupation(self._h, np.ascontiguousarray(W, np.float64), d))
+        return d
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sbdx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,276 @@
+// api.hip -- the extern "C" boundary of libsbdx (declared in include/sbdx.h).
+//
+// Every entry point converts C++ exceptions into a negative status plus a thread-local
+// message; host vectors are converted between the reference's interleaved basis and the
+// engine's blocked basis here and nowhere else.
+#include "common.h"
+#include "sbdx.h"
+
+#include <cstring>
+#include <string>
+
+namespace sbdx {
+void davidson_fp64(sbdx_ctx& c, const sbdx_davidson_opts& o, double* dW, sbdx_davidson_result& res,
+                   double* trace, double* resid, int trace_cap);
+void init_hf(sbdx_ctx& c, double* dW, hipStream_t st);
+void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody);
+void occupation_blocked(sbdx_ctx& c, const double* dW, double* density);
+}  // namespace sbdx
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    g_err.clear();
+    return SBDX_OK;
+  } catch (const sbdx::Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return SBDX_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return SBDX_ERR_ARG;
+  }
+}
+
+void need(bool ok, const char* msg) {
+  if (!ok) throw sbdx::Error(SBDX_ERR_ARG, msg);
+}
+
+void need_ints(const sbdx_ctx* c) {
+  if (!c) throw sbdx::Error(SBDX_ERR_ARG, "null context");
+  if (!c->have_ints) throw sbdx::Error(SBDX_ERR_STATE, "integrals not set (call sbdx_set_integrals)");
+}
+
+// host vector (interleaved basis) -> device (blocked basis) and back
+void upload_blocked(sbdx_ctx& c, const double* host, double* dev, sbdx::DevBuf<double>& tmp) {
+  const int64_t n = c.ndet();
+  tmp.ensure(n);
+  SBDX_HIP(hipMemcpyAsync(tmp.get(), host, n * 8, hipMemcpyHostToDevice, c.stream));
+  sbdx::apply_sign(c, SBDX_FP64, 1, tmp.get(), dev, c.stream);
+}
+
+void download_user(sbdx_ctx& c, const double* dev, double* host, sbdx::DevBuf<double>& tmp) {
+  const int64_t n = c.ndet();
+  tmp.ensure(n);
+  sbdx::apply_sign(c, SBDX_FP64, 1, dev, tmp.get(), c.stream);
+  SBDX_HIP(hipMemcpyAsync(host, tmp.get(), n * 8, hipMemcpyDeviceToHost, c.stream));
+  SBDX_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int sbdx_create(int norb, int64_t n_alpha, const uint64_t* alpha, int64_t n_beta,
+                const uint64_t* beta, sbdx_ctx** out) {
+  return guarded([&] {
+    need(out != nullptr, "out is null");
+    *out = nullptr;
+    need(norb >= 1 && norb <= 64, "norb must be in [1, 64]");
+    need(n_alpha >= 1 && n_beta >= 1, "empty string set");
+    need(n_alpha < (int64_t(1) << 32) && n_beta < (int64_t(1) << 32), "too many strings");
+    need(alpha && beta, "null string table");
+    const uint64_t mask = norb >= 64 ? ~0ull : ((1ull << norb) - 1ull);
+    for (int64_t i = 0; i < n_alpha; ++i) need((alpha[i] & ~mask) == 0, "alpha string has bits >= norb");
+    for (int64_t i = 0; i < n_beta; ++i) need((beta[i] & ~mask) == 0, "beta string has bits >= norb");
+    // every string of one spin must hold the same electron count: the excitation tables
+    // decode a popcount-2 (-4) difference as exactly one (two) electron(s) moved
+    for (int64_t i = 1; i < n_alpha; ++i)
+      need(sbdx::popc64(alpha[i]) == sbdx::popc64(alpha[0]), "alpha strings differ in electron count");
+    for (int64_t i = 1; i < n_beta; ++i)
+      need(sbdx::popc64(beta[i]) == sbdx::popc64(beta[0]), "beta strings differ in electron count");
+    auto* c = new sbdx_ctx();
+    try {
+      SBDX_HIP(hipGetDevice(&c->device));
+      SBDX_HIP(hipStreamCreate(&c->stream));
+      c->L = norb;
+      c->na = n_alpha;
+      c->nb = n_beta;
+      sbdx::build_tables(*c, alpha, beta);
+    } catch (...) {
+      if (c->stream) (void)hipStreamDestroy(c->stream);
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+int sbdx_set_integrals(sbdx_ctx* c, double I0, const double* I1, const double* I2, int64_t n_I2) {
+  return guarded([&] {
+    need(c && I1 && I2, "null argument");
+    const int64_t B = static_cast<int64_t>(c->L) * (c->L + 1) / 2;
+    need(n_I2 == B * (B + 1) / 2, "n_I2 must be B(B+1)/2 with B = norb(norb+1)/2");
+    c->I0 = I0;
+    c->h_I1.assign(I1, I1 + 4 * c->L * c->L);
+    c->h_I2.assign(I2, I2 + n_I2);
+    sbdx::build_integral_tables(*c);
+  });
+}
+
+int sbdx_excitation_counts(sbdx_ctx* c, int spin, int64_t* n_single, int64_t* n_double) {
+  return guarded([&] {
+    need(c && (spin == 0 || spin == 1), "bad argument");
+    const sbdx::SpinTab& t = spin == 0 ? c->A : c->B;
+    if (n_single) *n_single = t.nnz_s;
+    if (n_double) *n_double = t.nnz_d;
+  });
+}
+
+int sbdx_get_excitations(sbdx_ctx* c, int spin, int64_t* off_s, int64_t* idx_s, int64_t* off_d,
+                         int64_t* idx_d) {
+  return guarded([&] {
+    need(c && (spin == 0 || spin == 1), "bad argument");
+    const sbdx::SpinTab& t = spin == 0 ? c->A : c->B;
+    std::vector<uint32_t> tmp;
+    auto get = [&](const sbdx::DevBuf<int64_t>& off, const sbdx::DevBuf<uint32_t>& ket, int64_t nnz,
+                   int64_t* ho, int64_t* hi) {
+      if (ho) SBDX_HIP(hipMemcpy(ho, off.get(), (t.n + 1) * 8, hipMemcpyDeviceToHost));
+      if (hi && nnz) {
+        tmp.resize(nnz);
+        SBDX_HIP(hipMemcpy(tmp.data(), ket.get(), nnz * 4, hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < nnz; ++k) hi[k] = tmp[k];
+      }
+    };
+    get(t.off_s, t.ket_s, t.nnz_s, off_s, idx_s);
+    get(t.off_d, t.ket_d, t.nnz_d, off_d, idx_d);
+  });
+}
+
+int64_t sbdx_ndet(const sbdx_ctx* c) { return c ? c->ndet() : 0; }
+
+int sbdx_hdiag(sbdx_ctx* c, double* hii) {
+  return guarded([&] {
+    need_ints(c);
+    need(hii != nullptr, "null output");
+    SBDX_HIP(hipMemcpy(hii, c->hii.get(), c->ndet() * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int sbdx_sigma(sbdx_ctx* c, int nvec, const double* C, double* S) {
+  return guarded([&] {
+    need_ints(c);
+    need(C && S && nvec >= 1, "bad argument");
+    const int64_t n = c->ndet();
+    sbdx::DevBuf<double> x(n), y(n), tmp;
+    for (int v = 0; v < nvec; ++v) {
+      upload_blocked(*c, C + v * n, x.get(), tmp);
+      sbdx::sigma_blocked(*c, SBDX_FP64, 1, x.get(), y.get(), c->stream);
+      download_user(*c, y.get(), S + v * n, tmp);
+    }
+  });
+}
+
+int sbdx_sigma_device(sbdx_ctx* c, int prec, int nvec, const void* dC, void* dS, void* stream) {
+  return guarded([&] {
+    need_ints(c);
+    need(dC && dS && nvec >= 1 && (prec == SBDX_FP64 || prec == SBDX_FP32), "bad argument");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const int64_t n = c->ndet();
+    const size_t es = prec == SBDX_FP64 ? 8 : 4;
+    c->ws.tmp_in.ensure(n * es);
+    c->ws.tmp_out.ensure(n * es);
+    for (int v = 0; v < nvec; ++v) {
+      const char* xin = static_cast<const char*>(dC) + v * n * es;
+      char* yout = static_cast<char*>(dS) + v * n * es;
+      sbdx::apply_sign(*c, prec, 1, xin, c->ws.tmp_in.get(), st);
+      sbdx::sigma_blocked(*c, prec, 1, c->ws.tmp_in.get(), c->ws.tmp_out.get(), st);
+      sbdx::apply_sign(*c, prec, 1, c->ws.tmp_out.get(), yout, st);
+    }
+    SBDX_HIP(hipStreamSynchronize(st));
+  });
+}
+
+void sbdx_davidson_defaults(sbdx_davidson_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->max_iteration = 100;
+  o->num_block = 10;
+  o->eps = 1.0e-4;
+  o->max_time = 86400.0;
+  o->init = 0;
+  o->precision = SBDX_FP64;
+  o->nroots = 1;
+  o->preconditioner = 0;
+  o->verbose = 0;
+  o->energy_ref = 0.0;
+  o->energy_tol = 1.0e-6;
+}
+
+static void check_opts(const sbdx_davidson_opts* o) {
+  need(o != nullptr, "null options");
+  need(o->max_iteration >= 1 && o->num_block >= 1 && o->num_block <= 64, "bad iteration/block");
+  need(o->nroots == 1, "nroots > 1 is not available in this build");
+  need(o->precision == SBDX_FP64, "only FP64 Davidson is available in this build");
+}
+
+int sbdx_davidson_device(sbdx_ctx* c, const sbdx_davidson_opts* o, void* dW, sbdx_davidson_result* res,
+                         double* trace, double* resid, int trace_cap) {
+  return guarded([&] {
+    need_ints(c);
+    check_opts(o);
+    need(dW && res, "null argument");
+    std::memset(res, 0, sizeof(*res));
+    const int64_t n = c->ndet();
+    sbdx::DevBuf<double> w(n);
+    if (o->init == 0) sbdx::init_hf(*c, w.get(), c->stream);
+    else sbdx::apply_sign(*c, SBDX_FP64, 1, dW, w.get(), c->stream);
+    sbdx::davidson_fp64(*c, *o, w.get(), *res, trace, resid, trace_cap);
+    sbdx::apply_sign(*c, SBDX_FP64, 1, w.get(), dW, c->stream);
+    SBDX_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int sbdx_davidson(sbdx_ctx* c, const sbdx_davidson_opts* o, double* W, sbdx_davidson_result* res,
+                  double* trace, double* resid, int trace_cap) {
+  return guarded([&] {
+    need_ints(c);
+    check_opts(o);
+    need(W && res, "null argument");
+    std::memset(res, 0, sizeof(*res));
+    const int64_t n = c->ndet();
+    sbdx::DevBuf<double> w(n), tmp;
+    if (o->init == 0) sbdx::init_hf(*c, w.get(), c->stream);
+    else upload_blocked(*c, W, w.get(), tmp);
+    sbdx::davidson_fp64(*c, *o, w.get(), *res, trace, resid, trace_cap);
+    download_user(*c, w.get(), W, tmp);
+  });
+}
+
+int sbdx_rdm(sbdx_ctx* c, const double* W, double* onebody, double* twobody) {
+  return guarded([&] {
+    need_ints(c);
+    need(W && onebody && twobody, "null argument");
+    sbdx::DevBuf<double> w(c->ndet()), tmp;
+    upload_blocked(*c, W, w.get(), tmp);
+    sbdx::rdm_blocked(*c, w.get(), onebody, twobody);
+  });
+}
+
+int sbdx_occupation(sbdx_ctx* c, const double* W, double* density) {
+  return guarded([&] {
+    need(c && W && density, "null argument");
+    sbdx::DevBuf<double> w(c->ndet()), tmp;
+    upload_blocked(*c, W, w.get(), tmp);
+    sbdx::occupation_blocked(*c, w.get(), density);
+  });
+}
+
+void sbdx_destroy(sbdx_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  hipStream_t s = c->stream;
+  delete c;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+const char* sbdx_last_error(void) { return g_err.c_str(); }
+
+}  // extern "C"

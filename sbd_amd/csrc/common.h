@@ -1,0 +1,181 @@
+// common.h -- shared definitions of libsbdx (MI355X / gfx950 selected-basis engine).
+//
+// Working basis.  The reference orders spin-orbitals interleaved (alpha p -> 2p, beta p ->
+// 2p+1; chemistry/basic/determinants.h:98-119), so an alpha excitation's fermionic sign also
+// counts beta electrons.  The engine works in the spin-blocked order instead (all alpha
+// creators, then all beta): there every excitation sign depends on one spin string only, and
+//   H_interleaved = S H_blocked S,   S(ia,ib) = (-1)^{#{(p in alpha, q in beta) : q < p}},
+// a diagonal +-1.  Vectors are converted with S at the API boundary only; inside the Davidson
+// iteration every vector stays in the blocked basis (S is orthogonal and diagonal, so dot
+// products, norms and the diagonal preconditioner are unchanged).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sbdx {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SBDX_HIP(call)                                                                       \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      throw ::sbdx::Error(-3, std::string(#call) + ": " + hipGetErrorString(e_) + " at " +   \
+                                  __FILE__ + ":" + std::to_string(__LINE__));                \
+  } while (0)
+
+#define SBDX_CHECK_LAUNCH() SBDX_HIP(hipGetLastError())
+
+// Owning device allocation.
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    SBDX_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    n = count;
+  }
+  void ensure(size_t count) { if (n < count || p == nullptr) alloc(count); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  T* get() const { return p; }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Excitation tables of one spin, CSR over bra strings x, kets in ascending index order.
+// exc_s packs a single excitation: bits 0-5 annihilated orbital i (occupied in the ket),
+// bits 6-11 created orbital a (occupied in the bra), bit 12 set if the blocked sign is -1.
+struct SpinTab {
+  int64_t n = 0;
+  DevBuf<uint64_t> str;
+  DevBuf<int64_t> off_s, off_d;
+  DevBuf<uint32_t> ket_s, ket_d, exc_s;
+  DevBuf<double> val_s, val_d;   // singles: sign * (h_ai + same-spin sum); doubles: element
+  DevBuf<float> val_s32, val_d32;
+  DevBuf<double> e1;             // same-spin diagonal energy of each string
+  int64_t nnz_s = 0, nnz_d = 0;
+  int max_s = 0, max_d = 0;      // longest row of each list
+};
+
+struct Workspace {
+  DevBuf<double> partial;        // block partial sums of reductions
+  DevBuf<double> scal;           // device scalars
+  DevBuf<unsigned char> xT, yT;  // transposed copies for the beta part (any precision)
+  DevBuf<unsigned char> tmp_in, tmp_out;
+};
+
+}  // namespace sbdx
+
+struct sbdx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int L = 0;
+  int64_t na = 0, nb = 0;
+  sbdx::SpinTab A, B;
+  bool have_ints = false;
+  double I0 = 0.0;
+  std::vector<double> h_I1, h_I2;
+  sbdx::DevBuf<double> h1a, h1b;   // L x L one-electron blocks (alpha, beta)
+  sbdx::DevBuf<double> eri4;       // dense (pq|rs), index ((p L + q) L + r) L + s
+  sbdx::DevBuf<float> eri4_32;
+  sbdx::DevBuf<double> hii;        // H_ii, blocked and interleaved bases agree
+  sbdx::DevBuf<float> hii32;
+  sbdx::Workspace ws;
+  int64_t ndet() const { return na * nb; }
+};
+
+namespace sbdx {
+
+// ----------------------------------------------------------------------------- bit helpers
+
+__host__ __device__ inline int popc64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __popcll(x);
+#else
+  return __builtin_popcountll(x);
+#endif
+}
+
+__host__ __device__ inline int ctz64(uint64_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __ffsll(static_cast<unsigned long long>(x)) - 1;
+#else
+  return __builtin_ctzll(x);
+#endif
+}
+
+// bits in [lo, hi), 0 <= lo <= hi <= 64
+__host__ __device__ inline uint64_t mask_range(int lo, int hi) {
+  uint64_t up = hi >= 64 ? ~0ull : ((1ull << hi) - 1ull);
+  uint64_t dn = lo >= 64 ? ~0ull : ((1ull << lo) - 1ull);
+  return up & ~dn;
+}
+
+// Sign of a single excitation i -> a applied to ket string y (one spin, blocked order):
+// (-1)^{occupied orbitals strictly between i and a}.  Equals the reference's parity
+// (determinants.h:205-256, :408) evaluated on a one-spin determinant.
+__host__ __device__ inline int sign_single(uint64_t y, int i, int a) {
+  int lo = i < a ? i : a, hi = i < a ? a : i;
+  return (popc64(y & mask_range(lo + 1, hi)) & 1) ? -1 : 1;
+}
+
+// determinants.h:205-256 parity on a one-spin string: [st, en) popcount, flip if st set
+__host__ __device__ inline int parity_ref(uint64_t y, int st, int en) {
+  int s = (popc64(y & mask_range(st, en)) & 1) ? -1 : 1;
+  return ((y >> st) & 1ull) ? -s : s;
+}
+
+// determinants.h:424-442 on a one-spin string: I<J annihilated (ket), A<B created (bra)
+__host__ __device__ inline int sign_double(uint64_t y, int I, int J, int A, int B) {
+  int s = parity_ref(y, I < A ? I : A, I < A ? A : I);
+  s *= parity_ref(y, J < B ? J : B, J < B ? B : J);
+  if (A > J || B < I) s = -s;
+  return s;
+}
+
+// Interleaved <-> blocked sign of determinant (a, b).
+__host__ __device__ inline int det_sign(uint64_t a, uint64_t b) {
+  int c = 0;
+  while (a) {
+    int p = ctz64(a);
+    c += popc64(b & mask_range(0, p));
+    a &= a - 1;
+  }
+  return (c & 1) ? -1 : 1;
+}
+
+__host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ----------------------------------------------------------------------------- launchers
+
+void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta);  // tables.hip
+void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
+void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
+void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
+void apply_sign(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
+
+}  // namespace sbdx

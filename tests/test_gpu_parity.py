@@ -1,0 +1,109 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Integer work (excitation lists) must be bit-exact; floating point within the stated
+tolerances: H_ii and sigma 1e-12 relative (FP64), Davidson energies 1e-9 Ha.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from sbd_amd import synth
+from sbd_amd._lib import Engine, SbdxError
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # name, (L, na_el, nb_el, rank, n_alpha, n_beta)   None = full string space
+    ("l6_2a2b_full", (6, 2, 2, 4, None, None)),
+    ("l8_3a2b_full", (8, 3, 2, 6, None, None)),
+    ("l10_3a3b_samp", (10, 3, 3, 8, 60, 50)),
+    ("c2fam_300x280", (36, 8, 8, 72, 300, 280)),
+    ("c3fam_200x220", (48, 12, 12, 96, 200, 220)),
+]
+
+
+def make(case):
+    L, na, nb, R, Na, Nb = case
+    return synth.make_custom(L, na, nb, R, Na, Nb, seed_ints=3)
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[c[0] for c in CASES])
+def pair(request):
+    name, case = request.param
+    print(f"[case {name}]", flush=True)
+    inst = make(case)
+    return name, inst, O.System.from_instance(inst), Engine.from_instance(inst)
+
+
+def test_excitation_lists_bit_exact(pair):
+    _, inst, so, eng = pair
+    for spin, conn in ((0, so.conn_a), (1, so.conn_b)):
+        off_s, idx_s, off_d, idx_d = eng.excitations(spin)
+        assert np.array_equal(off_s, conn[0]) and np.array_equal(idx_s, conn[1])
+        assert np.array_equal(off_d, conn[2]) and np.array_equal(idx_d, conn[3])
+
+
+def test_hdiag(pair):
+    _, inst, so, eng = pair
+    ref = so.hdiag()
+    got = eng.hdiag()
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_sigma_fp64(pair, seed):
+    _, inst, so, eng = pair
+    x = np.random.default_rng(seed).standard_normal(so.ndet)
+    ref = so.mult(x)
+    got = eng.sigma(x)
+    rel = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert rel <= 1e-12, rel
+
+
+def test_sigma_fp32_device(pair):
+    torch = pytest.importorskip("torch")
+    _, inst, so, eng = pair
+    x = np.random.default_rng(7).standard_normal(so.ndet)
+    ref = so.mult(x)
+    dx = torch.tensor(x, dtype=torch.float32, device="cuda")
+    dy = torch.empty_like(dx)
+    eng.sigma_device(1, 1, dx.data_ptr(), dy.data_ptr())
+    got = dy.cpu().numpy().astype(np.float64)
+    rel = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert rel <= 2e-6, rel
+
+
+def test_davidson_energy(pair):
+    name, inst, so, eng = pair
+    if so.ndet > 20000:
+        pytest.skip("oracle Davidson too slow at this size on the CPU")
+    ref = so.davidson(max_it=100, num_block=10, eps=1e-8)
+    got = eng.davidson(max_iteration=100, num_block=10, eps=1e-8)
+    assert got["converged"]
+    assert abs(got["energy"] - ref["energy"]) < 1e-9
+    s = np.sign(np.dot(got["W"], ref["W"]))
+    assert np.max(np.abs(s * got["W"] - ref["W"])) < 1e-6
+
+
+def test_rdm(pair):
+    name, inst, so, eng = pair
+    if so.ndet > 20000:
+        pytest.skip("oracle RDM too slow at this size on the CPU")
+    W = np.random.default_rng(3).standard_normal(so.ndet)
+    W /= np.linalg.norm(W)
+    one_r, two_r = so.correlation(W)
+    one, two = eng.rdm(W)
+    assert np.max(np.abs(one - one_r)) < 1e-12
+    assert np.max(np.abs(two - two_r)) < 1e-12
+    d = eng.occupation(W)
+    assert np.max(np.abs(d - so.occupation(W))) < 1e-12
+
+
+def test_errors():
+    with pytest.raises(SbdxError):
+        Engine(65, np.array([1], np.uint64), np.array([1], np.uint64))
+    with pytest.raises(SbdxError):
+        Engine(4, np.array([1 << 5], np.uint64), np.array([1], np.uint64))
+    e = Engine(4, np.array([3, 5], np.uint64), np.array([3], np.uint64))
+    with pytest.raises(SbdxError):
+        e.hdiag()  # integrals not set
