@@ -125,6 +125,12 @@ int sbdx_rdm(sbdx_ctx *ctx, const double *W, double *onebody, double *twobody);
 /* Occupations density[2p + s] = <n_{p,s}> of W (host). */
 int sbdx_occupation(sbdx_ctx *ctx, const double *W, double *density);
 
+/* Measurement hook, no counterpart in the reference: when enabled, every sigma build records
+ * HIP events on its launch stream around its phases (0 alpha rows, 1 transpose, 2 beta rows,
+ * 3 transposed add) and accumulates their durations.  Enabling resets the sums. */
+int sbdx_profile(sbdx_ctx *ctx, int enable);
+int sbdx_profile_read(sbdx_ctx *ctx, double *ms, int cap, int64_t *calls);
+
 void sbdx_destroy(sbdx_ctx *ctx);
 
 /* Message of the last failed call on this host thread ("" if none). */

@@ -18,9 +18,10 @@ SBDX_FP32 = 1
 EXPORTS = [
     "sbdx_create", "sbdx_set_integrals", "sbdx_excitation_counts", "sbdx_get_excitations",
     "sbdx_ndet", "sbdx_hdiag", "sbdx_sigma", "sbdx_sigma_device", "sbdx_davidson_defaults",
-    "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_destroy",
-    "sbdx_last_error",
+    "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
+    "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
 ]
+PHASES = ["alpha_rows", "transpose", "beta_rows", "add_transposed"]
 
 
 class DavidsonOpts(C.Structure):
@@ -74,6 +75,8 @@ def lib():
                                        vp, vp, C.c_int]
     L.sbdx_rdm.argtypes = [vp, f64p, f64p, f64p]
     L.sbdx_occupation.argtypes = [vp, f64p, f64p]
+    L.sbdx_profile.argtypes = [vp, C.c_int]
+    L.sbdx_profile_read.argtypes = [vp, f64p, C.c_int, C.POINTER(C.c_int64)]
     L.sbdx_destroy.argtypes = [vp]
     L.sbdx_destroy.restype = None
     L.sbdx_last_error.restype = C.c_char_p
@@ -212,6 +215,15 @@ class Engine:
         d = np.zeros(2 * self.norb)
         check(lib().sbdx_occupation(self._h, np.ascontiguousarray(W, np.float64), d))
         return d
+
+    def profile(self, enable=True):
+        check(lib().sbdx_profile(self._h, int(enable)))
+
+    def profile_read(self):
+        ms = np.zeros(len(PHASES))
+        calls = C.c_int64()
+        check(lib().sbdx_profile_read(self._h, ms, ms.size, C.byref(calls)))
+        return dict(zip(PHASES, ms.tolist())), calls.value
 
     def close(self):
         if getattr(self, "_h", None):

@@ -263,8 +263,30 @@ int sbdx_occupation(sbdx_ctx* c, const double* W, double* density) {
   });
 }
 
+int sbdx_profile(sbdx_ctx* c, int enable) {
+  return guarded([&] {
+    need(c != nullptr, "null context");
+    sbdx::Prof& p = c->prof;
+    if (enable && !p.ev[0])
+      for (auto& e : p.ev) SBDX_HIP(hipEventCreate(&e));
+    p.on = enable != 0;
+    for (double& m : p.ms) m = 0.0;
+    p.calls = 0;
+  });
+}
+
+int sbdx_profile_read(sbdx_ctx* c, double* ms, int cap, int64_t* calls) {
+  return guarded([&] {
+    need(c && ms, "null argument");
+    for (int k = 0; k < cap && k < sbdx::kPhases; ++k) ms[k] = c->prof.ms[k];
+    if (calls) *calls = c->prof.calls;
+  });
+}
+
 void sbdx_destroy(sbdx_ctx* c) {
   if (!c) return;
+  for (auto& e : c->prof.ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   hipStream_t s = c->stream;
   delete c;

@@ -81,6 +81,15 @@ struct SpinTab {
   int max_s = 0, max_d = 0;      // longest row of each list
 };
 
+// Per-phase timing of sigma builds (HIP events on the launch stream), for measurement only.
+enum Phase { kPhaseAlpha = 0, kPhaseTranspose = 1, kPhaseBeta = 2, kPhaseAddT = 3, kPhases = 4 };
+struct Prof {
+  bool on = false;
+  hipEvent_t ev[kPhases + 1] = {};
+  double ms[kPhases] = {};
+  int64_t calls = 0;
+};
+
 struct Workspace {
   DevBuf<double> partial;        // block partial sums of reductions
   DevBuf<double> scal;           // device scalars
@@ -105,6 +114,7 @@ struct sbdx_ctx {
   sbdx::DevBuf<double> hii;        // H_ii, blocked and interleaved bases agree
   sbdx::DevBuf<float> hii32;
   sbdx::Workspace ws;
+  sbdx::Prof prof;
   int64_t ndet() const { return na * nb; }
 };
 

@@ -177,6 +177,11 @@ void sigma_one(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, con
   const int64_t n = na * nb;
   RowTabs ta{c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), c.A.off_d.get(), c.A.ket_d.get()};
   RowTabs tb{c.B.off_s.get(), c.B.ket_s.get(), c.B.exc_s.get(), c.B.off_d.get(), c.B.ket_d.get()};
+  Prof& pf = c.prof;
+  auto mark = [&](int k) {
+    if (pf.on) SBDX_HIP(hipEventRecord(pf.ev[k], st));
+  };
+  mark(0);
   // alpha rows: diagonal + alpha S/D + alpha-S x beta-S
   {
     const int64_t blocks = na * cdiv(nb, kTile);
@@ -184,6 +189,7 @@ void sigma_one(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, con
         na, nb, c.L, ta, vs_a, vd_a, c.B.str.get(), V, x, y, hii, tb);
     SBDX_CHECK_LAUNCH();
   }
+  mark(1);
   // beta rows on the transposed vector
   c.ws.xT.ensure(n * sizeof(T));
   c.ws.yT.ensure(n * sizeof(T));
@@ -192,14 +198,26 @@ void sigma_one(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, con
   const unsigned tblocks = static_cast<unsigned>(cdiv(na, 64) * cdiv(nb, 64));
   k_transpose<T><<<tblocks, 256, 0, st>>>(x, xT, na, nb);
   SBDX_CHECK_LAUNCH();
+  mark(2);
   {
     const int64_t blocks = nb * cdiv(na, kTile);
     k_sigma_rows<T, false><<<static_cast<unsigned>(blocks), kTile, 0, st>>>(
         nb, na, c.L, tb, vs_b, vd_b, c.A.str.get(), V, xT, yT, nullptr, ta);
     SBDX_CHECK_LAUNCH();
   }
+  mark(3);
   k_add_transposed<T><<<tblocks, 256, 0, st>>>(yT, y, na, nb);
   SBDX_CHECK_LAUNCH();
+  mark(4);
+  if (pf.on) {
+    SBDX_HIP(hipEventSynchronize(pf.ev[kPhases]));
+    for (int k = 0; k < kPhases; ++k) {
+      float ms = 0.0f;
+      SBDX_HIP(hipEventElapsedTime(&ms, pf.ev[k], pf.ev[k + 1]));
+      pf.ms[k] += ms;
+    }
+    pf.calls++;
+  }
 }
 
 }  // namespace
