@@ -126,8 +126,9 @@ int sbdx_rdm(sbdx_ctx *ctx, const double *W, double *onebody, double *twobody);
 int sbdx_occupation(sbdx_ctx *ctx, const double *W, double *density);
 
 /* Measurement hook, no counterpart in the reference: when enabled, every sigma build records
- * HIP events on its launch stream around its phases (0 alpha rows, 1 transpose, 2 beta rows,
- * 3 transposed add) and accumulates their durations.  Enabling resets the sums. */
+ * HIP events on its launch stream around its phases (0 alpha rows, 1 alpha-single x
+ * beta-single, 2 transpose, 3 beta rows, 4 transposed add) and accumulates their durations.
+ * Enabling resets the sums. */
 int sbdx_profile(sbdx_ctx *ctx, int enable);
 int sbdx_profile_read(sbdx_ctx *ctx, double *ms, int cap, int64_t *calls);
 

@@ -21,7 +21,7 @@ EXPORTS = [
     "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
     "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
 ]
-PHASES = ["alpha_rows", "transpose", "beta_rows", "add_transposed"]
+PHASES = ["alpha_rows", "mixed", "transpose", "beta_rows", "add_transposed"]
 
 
 class DavidsonOpts(C.Structure):

@@ -82,12 +82,26 @@ struct SpinTab {
 };
 
 // Per-phase timing of sigma builds (HIP events on the launch stream), for measurement only.
-enum Phase { kPhaseAlpha = 0, kPhaseTranspose = 1, kPhaseBeta = 2, kPhaseAddT = 3, kPhases = 4 };
+enum Phase { kPhaseAlpha = 0, kPhaseMixed = 1, kPhaseTranspose = 2, kPhaseBeta = 3, kPhaseAddT = 4,
+             kPhases = 5 };
 struct Prof {
   bool on = false;
   hipEvent_t ev[kPhases + 1] = {};
   double ms[kPhases] = {};
   int64_t calls = 0;
+};
+
+// Beta-single entries regrouped for the mixed (alpha single x beta single) pass: columns are
+// cut into tiles of consecutive beta strings holding at most kMixCap entries; inside a tile
+// entries are sorted by ket column jb.  meta = ib_local (bits 0-15) | pb = b*L + j (16-27) |
+// sign (bit 31).
+struct MixTiles {
+  int ntile = 0;
+  int max_cols = 0;
+  int ept = 32;              // entries per thread of the mixed kernel (16 or 32)
+  DevBuf<int64_t> ent_off;   // ntile + 1
+  DevBuf<int32_t> col0, ncol;
+  DevBuf<uint32_t> jb, meta;
 };
 
 struct Workspace {
@@ -105,6 +119,7 @@ struct sbdx_ctx {
   int L = 0;
   int64_t na = 0, nb = 0;
   sbdx::SpinTab A, B;
+  sbdx::MixTiles mix;
   bool have_ints = false;
   double I0 = 0.0;
   std::vector<double> h_I1, h_I2;
@@ -185,6 +200,7 @@ __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta);  // tables.hip
 void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
 void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
+void build_mix_tiles(sbdx_ctx& c);              // sigma.hip: mixed-pass tiles
 void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 void apply_sign(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 

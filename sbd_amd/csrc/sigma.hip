@@ -2,14 +2,14 @@
 //
 // sigma = D C + H_aa C + H_bb C + H_ab C, with C an n_alpha x n_beta row-major matrix.
 //
-//  * Row pass (alpha rows): one workgroup per (alpha row ia, tile of 256 beta columns); every
-//    wavefront covers 64 consecutive columns of the same row, so the row's excitation lists are
-//    wavefront-uniform (scalar loads) and every gathered operand C[ja, ib..ib+63] is one
-//    coalesced 512-byte (FP64) segment.  The diagonal, the alpha singles and doubles (type 2)
-//    and the alpha-single x beta-single part (type 0) are fused here, so sigma is written once,
-//    by its owner, with no atomics.
-//  * Beta part (type 1): C is transposed, the same row kernel runs with the beta tables, and
-//    the result is added back transposed.
+//  * Row pass (alpha rows, task type 2 + diagonal): one workgroup per (alpha row ia, tile of
+//    256 beta columns); every wavefront covers 64 consecutive columns of the same row, so the
+//    row's excitation lists are wavefront-uniform (scalar loads) and every gathered operand
+//    C[ja, ib..ib+63] is one coalesced 512-byte (FP64) segment.
+//  * Mixed pass (task type 0): k_sigma_mixed, described at the kernel.
+//  * Beta part (task type 1): C is transposed, the same row kernel runs with the beta tables,
+//    and the result is added back transposed.
+// Every pass updates sigma by its owner; there are no global atomics.
 //
 // Matrix elements (blocked order, signs from one spin string each):
 //   alpha single ja -> ia (i -> a):  s_a (F_a + sum_{j in beta(ib)} (ai|jj))
@@ -19,10 +19,15 @@
 // sum transformations, i.e. Ogita-Rump-Oishi Dot2 per output element).
 #include "common.h"
 
+#include <algorithm>
+#include <cstdlib>
+#include <utility>
+
 namespace sbdx {
 namespace {
 
 constexpr int kTile = 256;
+constexpr int kUnroll = 8;  // gathers issued back to back per wavefront in the row kernel
 
 struct AccD {
   double s = 0.0;
@@ -60,11 +65,13 @@ struct RowTabs {
   const uint32_t* ket_d;
 };
 
-template <typename T, bool kFuse>
+// Same-spin part (and, for the alpha rows, the diagonal): y[row, col] = sum over the row's
+// singles and doubles of value * x[ket, col].
+template <typename T, bool kDiag>
 __global__ __launch_bounds__(kTile) void k_sigma_rows(
     int64_t nrow, int64_t ncol, int L, RowTabs rt, const T* __restrict__ val_s,
     const T* __restrict__ val_d, const uint64_t* __restrict__ col_str, const T* __restrict__ V,
-    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ hii, RowTabs ct) {
+    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ hii) {
   const int64_t g = blockIdx.x;
   const int64_t tile = g / nrow;
   const int64_t row = g - tile * nrow;
@@ -73,19 +80,27 @@ __global__ __launch_bounds__(kTile) void k_sigma_rows(
   const int64_t L2 = static_cast<int64_t>(L) * L;
   typename AccOf<T>::type acc;
   const int64_t me = row * ncol + col;
-  if (kFuse) acc.fma(hii[me], x[me]);
+  if (kDiag) acc.fma(hii[me], x[me]);
 
-  // same-spin doubles
+  // same-spin doubles, kUnroll gathers in flight per wavefront
   const int64_t d0 = rt.off_d[row], d1 = rt.off_d[row + 1];
-  for (int64_t k = d0; k < d1; ++k) acc.fma(val_d[k], x[static_cast<int64_t>(rt.ket_d[k]) * ncol + col]);
+  const T* __restrict__ xc = x + col;
+  int64_t k = d0;
+  for (; k + kUnroll <= d1; k += kUnroll) {
+    T xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) xv[u] = xc[static_cast<int64_t>(rt.ket_d[k + u]) * ncol];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) acc.fma(val_d[k + u], xv[u]);
+  }
+  for (; k < d1; ++k) acc.fma(val_d[k], xc[static_cast<int64_t>(rt.ket_d[k]) * ncol]);
 
-  // same-spin singles, with the spectator string's Coulomb term
+  // same-spin singles, with the spectator string's Coulomb term sum_{j in col string} (ai|jj)
   const uint64_t ycol = col_str[col];
   const int64_t s0 = rt.off_s[row], s1 = rt.off_s[row + 1];
-  for (int64_t k = s0; k < s1; ++k) {
-    const uint32_t e = rt.exc_s[k];
-    const int i = e & 63, a = (e >> 6) & 63;
-    const T* vp = V + (static_cast<int64_t>(a) * L + i) * L2;
+  auto single_value = [&](int64_t q) {
+    const uint32_t e = rt.exc_s[q];
+    const T* vp = V + (static_cast<int64_t>((e >> 6) & 63) * L + (e & 63)) * L2;
     T coul = T(0);
     uint64_t o = ycol;
     while (o) {
@@ -93,29 +108,123 @@ __global__ __launch_bounds__(kTile) void k_sigma_rows(
       o &= o - 1;
       coul += vp[j * L + j];
     }
-    const T v = val_s[k] + ((e >> 12) & 1u ? -coul : coul);
-    acc.fma(v, x[static_cast<int64_t>(rt.ket_s[k]) * ncol + col]);
+    return val_s[q] + ((e >> 12) & 1u ? -coul : coul);
+  };
+  k = s0;
+  for (; k + kUnroll <= s1; k += kUnroll) {
+    T xv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) xv[u] = xc[static_cast<int64_t>(rt.ket_s[k + u]) * ncol];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) acc.fma(single_value(k + u), xv[u]);
   }
+  for (; k < s1; ++k) acc.fma(single_value(k), xc[static_cast<int64_t>(rt.ket_s[k]) * ncol]);
+  y[me] = acc.get();
+}
 
-  if (kFuse) {
-    // alpha single (row, wavefront-uniform) x beta single (column, per lane)
-    const int64_t m0 = ct.off_s[col], m1 = ct.off_s[col + 1];
-    for (int64_t k = s0; k < s1; ++k) {
-      const uint32_t e = rt.exc_s[k];
-      const int i = e & 63, a = (e >> 6) & 63;
-      const T* vp = V + (static_cast<int64_t>(a) * L + i) * L2;
-      const T* xr = x + static_cast<int64_t>(rt.ket_s[k]) * ncol;
-      typename AccOf<T>::type part;
-      for (int64_t m = m0; m < m1; ++m) {
-        const uint32_t eb = ct.exc_s[m];
-        const T v = vp[((eb >> 6) & 63) * L + (eb & 63)];
-        part.fma((eb >> 12) & 1u ? -v : v, xr[ct.ket_s[m]]);
-      }
-      const T p = part.get();
-      acc.add((e >> 12) & 1u ? -p : p);
+// Alpha single x beta single (task type 0, mult.h:569-596):
+//   y[ia, ib] += sum_{k in S_a(ia)} sum_{m in S_b(ib)} s_a s_b (a_k i_k | b_m j_m) x[ja_k, jb_m].
+// One workgroup = (alpha row ia, beta column tile).  The tile's beta-single entries (sorted by
+// jb) are held in registers for the whole loop over the row's alpha singles, so each entry is
+// loaded once per workgroup; for every alpha single the integral row V[(a_k i_k), :] (sign
+// folded in) is staged in double-buffered LDS, and the ket row x[ja_k, :] is gathered along
+// sorted jb.  Workgroups are numbered so that all tiles of one row run on the same XCD and
+// share that row's ket rows in L2.  Per-entry sums are folded into the tile's columns with LDS
+// atomics once at the end; sigma is updated by its owner, with no global atomics.
+constexpr int kMixThreads = 256;
+constexpr int kMixEPTDefault = 16;                // entries per thread (8, 16 or 32)
+constexpr int kMixMaxCols = 1024;
+// V row elements fetched per thread: 8 covers L^2 <= 2048 (L <= 45), 16 covers L <= 64
+
+template <typename T, int kMixEPT, int kVPerThread>
+__global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
+    int64_t na, int64_t nb, int L, const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket,
+    const uint32_t* __restrict__ a_exc, int ntile, const int64_t* __restrict__ t_off,
+    const int32_t* __restrict__ t_col0, const int32_t* __restrict__ t_ncol,
+    const uint32_t* __restrict__ e_jb, const uint32_t* __restrict__ e_meta, const T* __restrict__ V,
+    const T* __restrict__ x, T* __restrict__ y) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int L2 = L * L;
+  T* vs0 = reinterpret_cast<T*>(smem_raw);
+  T* vs1 = vs0 + L2;
+  T* out = vs1 + L2;
+
+  // XCD-aware numbering: workgroup g runs on XCD g % 8; XCD x takes rows ia = x (mod 8), all
+  // tiles of a row back to back
+  const int64_t g = blockIdx.x;
+  const int64_t xcd = g & 7, slot = g >> 3;
+  const int64_t tile = slot % ntile;
+  const int64_t ia = xcd + 8 * (slot / ntile);
+  if (ia >= na) return;
+  const int tid = threadIdx.x;
+  const int64_t k0 = a_off[ia], k1 = a_off[ia + 1];
+  if (k0 == k1) return;
+
+  const int64_t e0 = t_off[tile], e1 = t_off[tile + 1];
+  const int col0 = t_col0[tile], ncol = t_ncol[tile];
+  uint32_t jb[kMixEPT], meta[kMixEPT];
+  typename AccOf<T>::type acc[kMixEPT];
+  // padding entries (past the tile's end) read column 0 and pair 0, which are always valid, so
+  // the entry loop below has no branches and all its gathers can be in flight at once; their
+  // sums are dropped at the end (ib_local 0xffff)
+#pragma unroll
+  for (int j = 0; j < kMixEPT; ++j) {
+    const int64_t e = e0 + tid + static_cast<int64_t>(j) * kMixThreads;
+    const bool ok = e < e1;
+    jb[j] = ok ? e_jb[e] : 0u;
+    meta[j] = ok ? e_meta[e] : 0x0000ffffu;
+  }
+  for (int c = tid; c < ncol; c += kMixThreads) out[c] = T(0);
+
+  // V row of the next alpha single: fetched into registers before the current row's gathers
+  // (its latency hides under them), written to LDS after them
+  T pre[kVPerThread];
+  bool pre_neg = false;
+  auto fetch = [&](int64_t k) {
+    const uint32_t e = a_exc[k];
+    const T* src = V + static_cast<int64_t>(((e >> 6) & 63) * L + (e & 63)) * L2;
+    pre_neg = (e >> 12) & 1u;
+#pragma unroll
+    for (int u = 0; u < kVPerThread; ++u) {
+      const int p = tid + u * kMixThreads;
+      pre[u] = src[p < L2 ? p : 0];
+    }
+  };
+  auto commit = [&](T* dst) {
+#pragma unroll
+    for (int u = 0; u < kVPerThread; ++u) {
+      const int p = tid + u * kMixThreads;
+      if (p < L2) dst[p] = pre_neg ? -pre[u] : pre[u];
+    }
+  };
+  fetch(k0);
+  commit(vs0);
+  for (int64_t k = k0; k < k1; ++k) {
+    T* cur = ((k - k0) & 1) ? vs1 : vs0;
+    T* nxt = ((k - k0) & 1) ? vs0 : vs1;
+    __syncthreads();  // cur is staged; nobody reads nxt any more
+    const bool more = k + 1 < k1;
+    if (more) fetch(k + 1);
+    const T* __restrict__ xr = x + static_cast<int64_t>(a_ket[k]) * nb;
+    T xv[kMixEPT];
+#pragma unroll
+    for (int j = 0; j < kMixEPT; ++j) xv[j] = xr[jb[j]];
+#pragma unroll
+    for (int j = 0; j < kMixEPT; ++j) acc[j].fma(cur[(meta[j] >> 16) & 0xfff], xv[j]);
+    if (more) commit(nxt);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kMixEPT; ++j) {
+    const uint32_t m = meta[j];
+    if ((m & 0xffffu) != 0xffffu) {
+      const T v = acc[j].get();
+      atomicAdd(&out[m & 0xffffu], (m >> 31) ? -v : v);
     }
   }
-  y[me] = acc.get();
+  __syncthreads();
+  T* yr = y + ia * nb + col0;
+  for (int c = tid; c < ncol; c += kMixThreads) yr[c] += out[c];
 }
 
 // out[c * R + r] = in[r * C + c]
@@ -182,14 +291,34 @@ void sigma_one(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, con
     if (pf.on) SBDX_HIP(hipEventRecord(pf.ev[k], st));
   };
   mark(0);
-  // alpha rows: diagonal + alpha S/D + alpha-S x beta-S
+  // alpha rows: diagonal + alpha singles and doubles
   {
     const int64_t blocks = na * cdiv(nb, kTile);
     k_sigma_rows<T, true><<<static_cast<unsigned>(blocks), kTile, 0, st>>>(
-        na, nb, c.L, ta, vs_a, vd_a, c.B.str.get(), V, x, y, hii, tb);
+        na, nb, c.L, ta, vs_a, vd_a, c.B.str.get(), V, x, y, hii);
     SBDX_CHECK_LAUNCH();
   }
   mark(1);
+  // alpha single x beta single
+  if (c.mix.ntile > 0) {
+    const MixTiles& mt = c.mix;
+    const int64_t blocks = 8 * cdiv(na, 8) * mt.ntile;
+    const size_t lds = (2 * static_cast<size_t>(c.L) * c.L + kMixMaxCols) * sizeof(T);
+    auto launch = [&](auto kernel) {
+      if (lds > 65536)
+        SBDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+      kernel<<<static_cast<unsigned>(blocks), kMixThreads, lds, st>>>(
+          na, nb, c.L, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
+          mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), V, x, y);
+      SBDX_CHECK_LAUNCH();
+    };
+    const bool small_v = static_cast<int>(c.L) * c.L <= 8 * kMixThreads;
+    if (mt.ept == 8) small_v ? launch(&k_sigma_mixed<T, 8, 8>) : launch(&k_sigma_mixed<T, 8, 16>);
+    else if (mt.ept == 16) small_v ? launch(&k_sigma_mixed<T, 16, 8>) : launch(&k_sigma_mixed<T, 16, 16>);
+    else small_v ? launch(&k_sigma_mixed<T, 32, 8>) : launch(&k_sigma_mixed<T, 32, 16>);
+  }
+  mark(2);
   // beta rows on the transposed vector
   c.ws.xT.ensure(n * sizeof(T));
   c.ws.yT.ensure(n * sizeof(T));
@@ -198,17 +327,17 @@ void sigma_one(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, con
   const unsigned tblocks = static_cast<unsigned>(cdiv(na, 64) * cdiv(nb, 64));
   k_transpose<T><<<tblocks, 256, 0, st>>>(x, xT, na, nb);
   SBDX_CHECK_LAUNCH();
-  mark(2);
+  mark(3);
   {
     const int64_t blocks = nb * cdiv(na, kTile);
     k_sigma_rows<T, false><<<static_cast<unsigned>(blocks), kTile, 0, st>>>(
-        nb, na, c.L, tb, vs_b, vd_b, c.A.str.get(), V, xT, yT, nullptr, ta);
+        nb, na, c.L, tb, vs_b, vd_b, c.A.str.get(), V, xT, yT, nullptr);
     SBDX_CHECK_LAUNCH();
   }
-  mark(3);
+  mark(4);
   k_add_transposed<T><<<tblocks, 256, 0, st>>>(yT, y, na, nb);
   SBDX_CHECK_LAUNCH();
-  mark(4);
+  mark(5);
   if (pf.on) {
     SBDX_HIP(hipEventSynchronize(pf.ev[kPhases]));
     for (int k = 0; k < kPhases; ++k) {
@@ -221,6 +350,71 @@ void sigma_one(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, con
 }
 
 }  // namespace
+
+// Regroup the beta singles into mixed-pass tiles (host side, once per integral set: the packed
+// excitation words carry the orbital pair and sign).
+void build_mix_tiles(sbdx_ctx& c) {
+  const SpinTab& B = c.B;
+  const int L = c.L;
+  const int64_t nb = B.n;
+  MixTiles& mt = c.mix;
+  mt = MixTiles{};
+  // entries per thread: a tuning knob (register footprint vs. re-reads of ket rows per tile)
+  const char* env = std::getenv("SBDX_MIX_EPT");
+  const int want = env ? std::atoi(env) : kMixEPTDefault;
+  mt.ept = (want == 8 || want == 16) ? want : 32;
+  const int64_t kMixCap = static_cast<int64_t>(kMixThreads) * mt.ept;
+  if (B.nnz_s == 0 || c.A.nnz_s == 0) return;
+  std::vector<int64_t> off(nb + 1);
+  std::vector<uint32_t> ket(B.nnz_s), exc(B.nnz_s);
+  SBDX_HIP(hipMemcpy(off.data(), B.off_s.get(), (nb + 1) * 8, hipMemcpyDeviceToHost));
+  SBDX_HIP(hipMemcpy(ket.data(), B.ket_s.get(), B.nnz_s * 4, hipMemcpyDeviceToHost));
+  SBDX_HIP(hipMemcpy(exc.data(), B.exc_s.get(), B.nnz_s * 4, hipMemcpyDeviceToHost));
+  std::vector<int64_t> t_off{0};
+  std::vector<int32_t> t_col0, t_ncol;
+  std::vector<uint32_t> e_jb, e_meta;
+  e_jb.reserve(B.nnz_s);
+  e_meta.reserve(B.nnz_s);
+  std::vector<std::pair<uint32_t, uint32_t>> buf;
+  int64_t ib = 0;
+  while (ib < nb) {
+    const int64_t start = ib;
+    int64_t cnt = 0;
+    while (ib < nb && ib - start < kMixMaxCols && cnt + (off[ib + 1] - off[ib]) <= kMixCap) {
+      cnt += off[ib + 1] - off[ib];
+      ++ib;
+    }
+    if (ib == start) throw Error(-1, "beta string with more singles than a mixed tile holds");
+    buf.clear();
+    for (int64_t b = start; b < ib; ++b)
+      for (int64_t k = off[b]; k < off[b + 1]; ++k) {
+        const uint32_t e = exc[k];
+        const uint32_t pb = ((e >> 6) & 63) * static_cast<uint32_t>(L) + (e & 63);
+        const uint32_t meta = static_cast<uint32_t>(b - start) | (pb << 16) | (((e >> 12) & 1u) << 31);
+        buf.emplace_back(ket[k], meta);
+      }
+    std::stable_sort(buf.begin(), buf.end(), [](const auto& u, const auto& v) { return u.first < v.first; });
+    for (auto& p : buf) {
+      e_jb.push_back(p.first);
+      e_meta.push_back(p.second);
+    }
+    t_col0.push_back(static_cast<int32_t>(start));
+    t_ncol.push_back(static_cast<int32_t>(ib - start));
+    t_off.push_back(static_cast<int64_t>(e_jb.size()));
+    mt.max_cols = std::max<int>(mt.max_cols, static_cast<int>(ib - start));
+  }
+  mt.ntile = static_cast<int>(t_col0.size());
+  mt.ent_off.alloc(t_off.size());
+  mt.col0.alloc(t_col0.size());
+  mt.ncol.alloc(t_ncol.size());
+  mt.jb.alloc(e_jb.size());
+  mt.meta.alloc(e_meta.size());
+  SBDX_HIP(hipMemcpy(mt.ent_off.get(), t_off.data(), t_off.size() * 8, hipMemcpyHostToDevice));
+  SBDX_HIP(hipMemcpy(mt.col0.get(), t_col0.data(), t_col0.size() * 4, hipMemcpyHostToDevice));
+  SBDX_HIP(hipMemcpy(mt.ncol.get(), t_ncol.data(), t_ncol.size() * 4, hipMemcpyHostToDevice));
+  SBDX_HIP(hipMemcpy(mt.jb.get(), e_jb.data(), e_jb.size() * 4, hipMemcpyHostToDevice));
+  SBDX_HIP(hipMemcpy(mt.meta.get(), e_meta.data(), e_meta.size() * 4, hipMemcpyHostToDevice));
+}
 
 void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t st) {
   if (!c.have_ints) throw Error(-2, "integrals not set");
