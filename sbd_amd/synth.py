@@ -84,8 +84,9 @@ def make_integrals(norb: int, rank: int, seed: int) -> Integrals:
     a = rng.normal(0.0, 0.1, size=(rank, L, L))
     lk = np.triu(a) + np.transpose(np.triu(a, 1), (0, 2, 1))
     eri = np.einsum("kpq,krs->pqrs", lk, lk, optimize=True)
-    # exact 8-fold symmetry (the einsum is symmetric up to rounding of the sum order)
-    eri = 0.5 * (eri + eri.transpose(2, 3, 0, 1))
+    # exact 8-fold symmetry: every element takes the value of its canonical representative
+    # (the einsum is symmetric only up to the rounding of its summation order)
+    eri = unpack_eri(pack_eri(eri), L)
     return Integrals(norb=L, h1=h1, eri=eri, ecore=0.0)
 
 

@@ -1,0 +1,60 @@
+"""The C ABI library loads and exports every entry point include/sbdx.h declares, and its
+host-side argument checks reject bad input before any GPU call (CPU-only: no compute)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from sbd_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sbd[xo]_\w+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = C.CDLL(_lib.LIB_PATH)
+    names = declared("sbdx.h")
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(_lib.EXPORTS)
+
+
+def test_oracle_exports_its_header():
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "libsbdo.so"))
+    text = open(os.path.join(ROOT, "oracle", "sbdo.h")).read()
+    names = sorted(set(re.findall(r"\b(sbdo_\w+)\s*\(", text)))
+    assert names and all(hasattr(lib, n) for n in names)
+
+
+def test_defaults_match_reference_struct():
+    o = _lib.davidson_defaults()
+    # chemistry/tpb/sbdiag.h:12-30: max_nb 10, eps 1e-4, max_time 86400, init 0
+    assert o.num_block == 10 and o.eps == 1e-4 and o.max_time == 86400.0 and o.init == 0
+    assert o.nroots == 1 and o.precision == _lib.SBDX_FP64
+
+
+@pytest.mark.parametrize("norb,alpha,beta,msg", [
+    (65, [1], [1], "norb"),
+    (0, [1], [1], "norb"),
+    (4, [1 << 5], [1], "bits >= norb"),
+    (4, [3, 1], [3], "electron count"),
+    (4, [3], [3, 5, 1], "electron count"),
+])
+def test_create_rejects_bad_input_on_the_host(norb, alpha, beta, msg):
+    with pytest.raises(_lib.SbdxError) as e:
+        _lib.Engine(norb, np.array(alpha, np.uint64), np.array(beta, np.uint64))
+    assert e.value.code == -1 and msg in str(e.value)
+
+
+def test_null_arguments():
+    lib = _lib.lib()
+    assert lib.sbdx_create(4, 1, np.array([3], np.uint64), 1, np.array([3], np.uint64), None) == -1
+    assert lib.sbdx_ndet(None) == 0

@@ -1,0 +1,54 @@
+"""The seeded workload generator: deterministic, and the recipe it states."""
+import math
+
+import numpy as np
+
+from sbd_amd import synth
+
+
+def test_integrals_recipe():
+    ints = synth.make_integrals(12, 24, seed=0)
+    h, eri = ints.h1, ints.eri
+    assert np.allclose(h, h.T)
+    d = np.diag(h)
+    assert np.all(np.diff(d) >= 0) and d.min() >= -2 and d.max() <= 1
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.allclose(eri, eri.transpose(perm), atol=1e-14)
+    M = eri.reshape(144, 144)
+    assert np.linalg.eigvalsh(M).min() > -1e-10           # sum of squares of L^k
+
+
+def test_generator_deterministic():
+    a = synth.make_instance("c2", strings=500)
+    b = synth.make_instance("c2", strings=500)
+    assert np.array_equal(a.alpha, b.alpha) and np.array_equal(a.beta, b.beta)
+    assert np.array_equal(a.ints.eri, b.ints.eri)
+    assert not np.array_equal(a.alpha, a.beta)
+
+
+def test_strings():
+    s = synth.full_space(12, 4)
+    assert s.size == math.comb(12, 4) == 495 and np.all(np.diff(s.astype(np.int64)) > 0)
+    inst = synth.make_instance("c3", strings=700)
+    for t in (inst.alpha, inst.beta):
+        assert t.size == 700 and np.unique(t).size == 700
+        assert np.all(np.diff(t.astype(np.int64)) > 0)
+        assert all(bin(int(x)).count("1") == 12 for x in t)
+        assert int(t.max()) < (1 << 48)
+
+
+def test_packed_eri_round_trip():
+    ints = synth.make_integrals(7, 5, seed=3)
+    store = ints.packed_eri()
+    B = 7 * 8 // 2
+    assert store.size == B * (B + 1) // 2
+    assert np.array_equal(synth.unpack_eri(store, 7), ints.eri)
+
+
+def test_fcidump_entries_cover_store():
+    ints = synth.make_integrals(5, 4, seed=1)
+    vals, idx = ints.fcidump_entries()
+    two = idx[:, 2] > 0
+    B = 15
+    assert two.sum() == B * (B + 1) // 2
+    assert ((idx[:, 2] == 0) & (idx[:, 0] > 0)).sum() == 15
