@@ -43,6 +43,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+PHASE_KERNELS = {
+    "alpha_rows": "k_sigma_rows<alpha> (diagonal + alpha singles/doubles)",
+    "mixed": "k_sigma_mixed (alpha single x beta single)",
+    "transpose": "k_transpose",
+    "beta_rows": "k_sigma_rows<beta> (beta singles/doubles, transposed)",
+    "add_transposed": "k_add_transposed",
+}
+
+
+def measured_traffic(phase, workload, strings, prec):
+    """HBM-side bytes per launch of a phase's kernel, from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, calibrated on a plain 1 GiB copy), when one exists
+    for this workload and precision; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if strings is not None or not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get(f"{workload}/{prec}", {}).get(phase)
+    return None if rec is None else rec["bytes_per_launch"]
+
+
 def nnz_parts(eng):
     """Exact off-diagonal element counts of one sigma (same-spin alpha, same-spin beta, mixed)."""
     sa, da = eng.excitation_counts(0)
@@ -158,10 +178,18 @@ def main():
     # algorithmic bytes: one gathered word per off-diagonal element, plus the read of C and
     # write of sigma (2N words) and the FP64 diagonal (8N bytes)
     sigma_bytes = w * (nnz + 2 * N) + 8 * N
-    alpha_bytes = w * (parts["aa"] + parts["ab"] + 2 * N) + 8 * N
-    alpha_ms = phases["alpha_rows"] / max(calls, 1)
-    alpha_gbs = alpha_bytes / (alpha_ms * 1e-3) / 1e9
+    phase_bytes = {
+        "alpha_rows": w * (parts["aa"] + 2 * N) + 8 * N,     # gathers, x and y, H_ii
+        "mixed": w * (parts["ab"] + 2 * N),                  # gathers, read-modify-write of y
+        "transpose": 2 * w * N,
+        "beta_rows": w * (parts["bb"] + 2 * N),
+        "add_transposed": 3 * w * N,
+    }
+    phase_ms = {k: v / max(calls, 1) for k, v in phases.items()}
+    dom = max(phase_ms, key=phase_ms.get)
+    dom_gbs = phase_bytes[dom] / (phase_ms[dom] * 1e-3) / 1e9
     sigma_gbs = sigma_bytes / sec / 1e9
+    traffic = measured_traffic(dom, args.workload, args.strings, args.prec)
     out = {
         "metric": METRIC,
         "value": round(value, 6),
@@ -184,15 +212,15 @@ def main():
             "precision": args.prec,
         },
         "roofline": {
-            "kernel": "k_sigma_rows<alpha, fused> (alpha S/D + alpha x beta S + diagonal)",
+            "kernel": PHASE_KERNELS[dom],
             "bound": "hbm",
-            "achieved": round(alpha_gbs, 1),
+            "achieved": round(dom_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(alpha_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "algorithmic_bytes": alpha_bytes,
-            "mean_ms": round(alpha_ms, 3),
+            "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": phase_bytes[dom],
+            "mean_ms": round(phase_ms[dom], 3),
         },
         "sigma_roofline": {
             "bound": "hbm",
