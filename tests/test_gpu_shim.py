@@ -1,0 +1,47 @@
+"""The drop-in C++ header API (include/sbd_amd/sbd.h): a C++ program drives sbd::tpb::diag,
+sbd::makeQChamDiagTerms, sbd::mult and sbd::Correlation through it, as the reference's sample
+driver does, and its outputs are checked against the CPU oracle."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from sbd_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "shim_diag")
+
+
+def test_cpp_drop_in_api(tmp_path):
+    if not os.path.exists(BIN):
+        pytest.skip("tests/cpp/shim_diag not built (needs the image's MPI)")
+    inst = synth.make_custom(10, 3, 3, 8, 60, 50, seed_ints=3)
+    vals, idx = inst.ints.fcidump_entries()
+    path = str(tmp_path / "inst.bin")
+    O.write_instance(path, inst.norb, 3, 3, inst.alpha, inst.beta, vals, idx)
+    out = str(tmp_path / "shim")
+    r = subprocess.run([BIN, path, out, "1e-8"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    so = O.System.from_instance(inst)
+    ref = so.davidson(max_it=200, num_block=10, eps=1e-8)
+    assert abs(res["energy"] - ref["energy"]) < 1e-9
+    assert " Energy = " in r.stdout and "Davidson iteration" in r.stdout
+    hii = np.fromfile(out + ".hii")
+    assert np.max(np.abs(hii - so.hdiag())) < 1e-12 * np.abs(hii).max()
+    x = np.cos(0.37 * np.arange(so.ndet))
+    y = np.fromfile(out + ".multy")
+    want = 1.0 + so.mult(x)
+    assert np.max(np.abs(y - want)) < 1e-12 * np.abs(want).max()
+    one, two = so.correlation(ref["W"])
+    assert np.max(np.abs(np.fromfile(out + ".one") - one.ravel())) < 1e-6
+    assert np.max(np.abs(np.fromfile(out + ".two") - two.ravel())) < 1e-6
+    assert np.max(np.abs(np.fromfile(out + ".density") - so.occupation(ref["W"]))) < 1e-6
+    assert res["carryover"] >= 1
+    wf = out + ".wf000000"
+    assert os.path.getsize(wf) == 3 * 8 + (60 + 50) * 8 + so.ndet * 8
