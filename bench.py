@@ -38,6 +38,28 @@ METRIC = ("time-to-1e-6 Ha & sigma builds/s at 4e8 dets, 1 GPU: plain FP64 vs "
           "mixed+block+MLP")
 HBM_PEAK_GBS = 8000.0
 
+# Davidson variants of --solve: the reference algorithm in FP64, and the block thick-restart
+# solver in FP64 or mixed precision (FP32 corrections, FP64 anchor) with 1 or 2 vectors a block
+SOLVERS = {
+    "plain": dict(method=0, num_block=10),
+    "block_fp64": dict(method=1, precision=0, block=1, num_block=10),
+    "mixed": dict(method=1, precision=1, block=1, num_block=10),
+    "mixed_block2": dict(method=1, precision=1, block=2, num_block=8),
+    "mixed_block4": dict(method=1, precision=1, block=4, num_block=6),
+}
+
+
+def reference_energy(workload, strings):
+    """Converged FP64 ground-state energy of the workload (profiles/energies.json), or None."""
+    path = os.path.join(ROOT, "profiles", "energies.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    key = workload if strings is None else f"{workload}/{strings}"
+    e = table.get(key)
+    return None if e is None else float(e["energy"])
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -108,11 +130,15 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--strings", type=int, default=None, help="reduced string count (tests)")
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--solve", action="store_true", help="also time Davidson to 1e-6 Ha")
+    ap.add_argument("--solve", nargs="*", default=None, choices=sorted(SOLVERS),
+                    help="also time these Davidson variants to 1e-6 Ha of the reference energy")
     ap.add_argument("--solve-time", type=float, default=240.0)
+    ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain 1 GiB device copy (rocprof traffic calibration)")
+    ap.add_argument("--block", type=int, nargs="*", default=[],
+                    help="also time the fused sigma of blocks of these widths (1, 2, 4, 8)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -234,18 +260,62 @@ def main():
         "setup_s": round(t_setup, 3),
     }
 
-    if args.solve:
+    if args.block:
         del x, y
         torch.cuda.empty_cache()
-        W = torch.empty(N, dtype=torch.float64, device="cuda")
-        r = eng.davidson_device(W.data_ptr(), max_iteration=1000, num_block=10, eps=1e-8,
-                                max_time=args.solve_time, verbose=0)
-        tr = r["trace"]
-        e_best = float(tr.min()) if tr.size else float("nan")
-        out["solve"] = dict(steps=r["steps"], seconds=round(r["seconds"], 3),
-                            converged=r["converged"], residual=r["residual"],
-                            energy=r["energy"], min_ritz=e_best)
-        del W
+        out["block_sigma"] = {}
+        for k in args.block:
+            xb = torch.randn(N * k, dtype=dt, device="cuda", generator=g)
+            yb = torch.empty_like(xb)
+            eng.sigma_block_device(prec, k, xb.data_ptr(), yb.data_ptr(), stream)
+            torch.cuda.synchronize()
+            eng.profile(True)
+            t0 = time.perf_counter()
+            reps = max(1, args.steps // 2)
+            for _ in range(reps):
+                eng.sigma_block_device(prec, k, xb.data_ptr(), yb.data_ptr(), stream)
+            torch.cuda.synchronize()
+            tb = (time.perf_counter() - t0) / reps
+            ph, cl = eng.profile_read()
+            eng.profile(False)
+            out["block_sigma"][str(k)] = {
+                "ms_per_block": round(tb * 1e3, 3),
+                "vector_sigmas_per_s": round(k / tb, 4),
+                "phase_ms": {p: round(v / max(cl, 1), 3) for p, v in ph.items()},
+            }
+            log(f"[bench] block {k}: {tb * 1e3:.1f} ms, {k / tb:.3f} vector sigmas/s")
+            del xb, yb
+            torch.cuda.empty_cache()
+        x = y = None
+
+    if args.solve is not None:
+        names = args.solve or ["plain", "mixed_block2"]
+        del x, y
+        torch.cuda.empty_cache()
+        e_ref = reference_energy(args.workload, args.strings)
+        out["solve"] = {"energy_ref": e_ref, "energy_tol": 1e-6}
+        for name in names:
+            opts = dict(SOLVERS[name], max_iteration=100000, eps=1e-8, max_time=args.solve_time,
+                        verbose=int(args.verbose))
+            if e_ref is not None:
+                opts.update(energy_ref=e_ref, energy_tol=1e-6, stop_at_tol=1)
+            k = max(1, opts.get("nroots", 1))
+            W = torch.empty(N * k, dtype=torch.float64, device="cuda")
+            log(f"[bench] solve {name}: {opts}")
+            r = eng.davidson_device(W.data_ptr(), **opts)
+            tr = r["trace"]
+            out["solve"][name] = dict(
+                seconds_to_tol=r["seconds_to_tol"], steps_to_tol=r["steps_to_tol"],
+                steps=r["steps"], seconds=round(r["seconds"], 3), converged=r["converged"],
+                residual=r["residual"], energy=r["energy"],
+                min_ritz=float(tr.min()) if tr.size else None)
+            log(f"[bench] solve {name}: {out['solve'][name]}")
+            del W
+            torch.cuda.empty_cache()
+        t = {n: out["solve"][n]["seconds_to_tol"] for n in names}
+        if "plain" in t and t["plain"] > 0:
+            out["solve"]["speedup_vs_plain"] = {n: round(t["plain"] / v, 3) for n, v in t.items()
+                                                if v > 0 and n != "plain"}
 
     if not args.no_cpu_baseline:
         cb = cpu_baseline(args.workload if args.workload in ("c2", "c3") else "c2")

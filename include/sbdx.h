@@ -80,20 +80,35 @@ int sbdx_sigma(sbdx_ctx *ctx, int nvec, const double *C, double *S);
 int sbdx_sigma_device(sbdx_ctx *ctx, int prec, int nvec, const void *dC, void *dS,
                       void *stream);
 
+/* Fused sigma of a block of k vectors (k = 1, 2, 4 or 8) stored interleaved: element
+ * [det * k + v] of dC / dS (device pointers, type by prec).  One pass over the excitation lists
+ * serves all k vectors (mult, chemistry/tpb/mult.h:387-648, applied to k vectors at once). */
+int sbdx_sigma_block_device(sbdx_ctx *ctx, int prec, int k, const void *dC, void *dS,
+                            void *stream);
+
 /* Davidson options (defaults by sbdx_davidson_defaults = the reference's SBD struct,
  * chemistry/tpb/sbdiag.h:12-30, with the iteration cap raised to converge). */
 typedef struct {
   int max_iteration;      /* restarts (reference: max_it) */
-  int num_block;          /* subspace size per restart (reference: max_nb) */
-  double eps;             /* residual tolerance ||E0 W - H W|| */
+  int num_block;          /* subspace size per restart, in blocks (reference: max_nb) */
+  double eps;             /* residual tolerance ||E0 W - H W||, every requested root */
   double max_time;        /* wall-clock budget in seconds */
-  int init;               /* 0: W[0] = 1 (HF, reference init 0); 1: use W as given */
-  int precision;          /* SBDX_FP64 or SBDX_FP32 (mixed: FP32 sigma, FP64 refinement) */
-  int nroots;             /* lowest roots solved together (block Davidson), >= 1 */
-  int preconditioner;     /* 0: diagonal (Jacobi); 1: learned (falls back to 0 on stall) */
+  int init;               /* 0: W[0] = 1 (HF, reference init 0); 1: vector 0 of W as given */
+  int precision;          /* SBDX_FP64, or SBDX_FP32 = mixed precision (method 1 only): FP32
+                             correction blocks and sigmas, FP64 anchor sigma, Rayleigh-Ritz,
+                             orthogonalization and residual */
+  int nroots;             /* lowest roots returned and converged, 1..8 (> 1: method 1) */
+  int preconditioner;     /* 0: diagonal (Jacobi); 1: learned (method 1; falls back to 0 on
+                             stall) */
   int verbose;            /* print the reference's per-step line */
-  double energy_ref;      /* if nonzero: record time to |E - energy_ref| < energy_tol */
+  double energy_ref;      /* if nonzero: record time to |E0 - energy_ref| < energy_tol, judged
+                             on FP64 Ritz values only */
   double energy_tol;
+  int method;             /* 0: the reference's algorithm (one vector, restart to the Ritz
+                             vector every num_block steps); 1: block thick-restart Davidson */
+  int block;              /* method 1: vectors per block (1, 2, 4, 8; rounded up to a power of
+                             two >= nroots); 0 = nroots */
+  int stop_at_tol;        /* 1: stop once energy_ref is reached (time-to-solution runs) */
 } sbdx_davidson_opts;
 
 typedef struct {
@@ -108,8 +123,10 @@ typedef struct {
 
 void sbdx_davidson_defaults(sbdx_davidson_opts *opts);
 
-/* W: host buffer of nroots*ndet doubles (in: initial guess if init=1; out: Ritz vectors).
- * trace/resid (optional, may be NULL): lowest Ritz value and residual norm per step. */
+/* W: host buffer of nroots*ndet doubles, vector-major (in: initial guess in vector 0 if
+ * init=1; out: the nroots lowest Ritz vectors).  trace/resid (optional, may be NULL): lowest
+ * Ritz value and residual norm per step.  Returns SBDX_OK also when the iteration stops before
+ * eps (res->converged = 0). */
 int sbdx_davidson(sbdx_ctx *ctx, const sbdx_davidson_opts *opts, double *W,
                   sbdx_davidson_result *res, double *trace, double *resid, int trace_cap);
 

@@ -17,7 +17,8 @@ SBDX_FP32 = 1
 
 EXPORTS = [
     "sbdx_create", "sbdx_set_integrals", "sbdx_excitation_counts", "sbdx_get_excitations",
-    "sbdx_ndet", "sbdx_hdiag", "sbdx_sigma", "sbdx_sigma_device", "sbdx_davidson_defaults",
+    "sbdx_ndet", "sbdx_hdiag", "sbdx_sigma", "sbdx_sigma_device", "sbdx_sigma_block_device",
+    "sbdx_davidson_defaults",
     "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
     "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
 ]
@@ -28,7 +29,8 @@ class DavidsonOpts(C.Structure):
     _fields_ = [("max_iteration", C.c_int), ("num_block", C.c_int), ("eps", C.c_double),
                 ("max_time", C.c_double), ("init", C.c_int), ("precision", C.c_int),
                 ("nroots", C.c_int), ("preconditioner", C.c_int), ("verbose", C.c_int),
-                ("energy_ref", C.c_double), ("energy_tol", C.c_double)]
+                ("energy_ref", C.c_double), ("energy_tol", C.c_double), ("method", C.c_int),
+                ("block", C.c_int), ("stop_at_tol", C.c_int)]
 
 
 class DavidsonResult(C.Structure):
@@ -67,6 +69,7 @@ def lib():
     L.sbdx_hdiag.argtypes = [vp, f64p]
     L.sbdx_sigma.argtypes = [vp, C.c_int, f64p, f64p]
     L.sbdx_sigma_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    L.sbdx_sigma_block_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
     L.sbdx_davidson_defaults.argtypes = [C.POINTER(DavidsonOpts)]
     L.sbdx_davidson_defaults.restype = None
     L.sbdx_davidson.argtypes = [vp, C.POINTER(DavidsonOpts), f64p, C.POINTER(DavidsonResult),
@@ -172,13 +175,21 @@ class Engine:
         check(lib().sbdx_sigma_device(self._h, prec, nvec, C.c_void_p(dC), C.c_void_p(dS),
                                       C.c_void_p(stream) if stream else None))
 
+    def sigma_block_device(self, prec, k, dC, dS, stream=None):
+        """Fused sigma of k interleaved vectors (element [det * k + v])."""
+        check(lib().sbdx_sigma_block_device(self._h, prec, k, C.c_void_p(dC), C.c_void_p(dS),
+                                            C.c_void_p(stream) if stream else None))
+
     def davidson(self, W=None, trace_cap=4096, **opts):
         o = davidson_defaults(**opts)
+        nr = max(1, o.nroots)
         if W is None:
-            W = np.zeros(self.ndet * max(1, o.nroots))
+            W = np.zeros(self.ndet * nr)
             o.init = 0
         else:
-            W = np.ascontiguousarray(W, dtype=np.float64).copy()
+            W0 = np.ascontiguousarray(W, dtype=np.float64).ravel()
+            W = np.zeros(self.ndet * nr)
+            W[:W0.size] = W0[:W.size]
             o.init = 1
         res = DavidsonResult()
         trace = np.zeros(trace_cap)

@@ -13,6 +13,9 @@ namespace sbdx {
 void davidson_fp64(sbdx_ctx& c, const sbdx_davidson_opts& o, double* dW, sbdx_davidson_result& res,
                    double* trace, double* resid, int trace_cap);
 void init_hf(sbdx_ctx& c, double* dW, hipStream_t st);
+void davidson_block(sbdx_ctx& c, const sbdx_davidson_opts& o, double* dW, sbdx_davidson_result& res,
+                    double* trace, double* resid, int trace_cap);
+int davidson_block_width(const sbdx_davidson_opts& o);
 void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody);
 void occupation_blocked(sbdx_ctx& c, const double* dW, double* density);
 }  // namespace sbdx
@@ -188,6 +191,23 @@ int sbdx_sigma_device(sbdx_ctx* c, int prec, int nvec, const void* dC, void* dS,
   });
 }
 
+int sbdx_sigma_block_device(sbdx_ctx* c, int prec, int k, const void* dC, void* dS, void* stream) {
+  return guarded([&] {
+    need_ints(c);
+    need(dC && dS && (k == 1 || k == 2 || k == 4 || k == 8) && (prec == SBDX_FP64 || prec == SBDX_FP32),
+         "bad argument (k must be 1, 2, 4 or 8)");
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    const int64_t n = c->ndet();
+    const size_t es = prec == SBDX_FP64 ? 8 : 4;
+    c->ws.tmp_in.ensure(n * k * es);
+    c->ws.tmp_out.ensure(n * k * es);
+    sbdx::apply_sign_block(*c, prec, k, dC, c->ws.tmp_in.get(), st);
+    sbdx::sigma_block(*c, prec, k, c->ws.tmp_in.get(), c->ws.tmp_out.get(), st);
+    sbdx::apply_sign_block(*c, prec, k, c->ws.tmp_out.get(), dS, st);
+    SBDX_HIP(hipStreamSynchronize(st));
+  });
+}
+
 void sbdx_davidson_defaults(sbdx_davidson_opts* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
@@ -202,28 +222,50 @@ void sbdx_davidson_defaults(sbdx_davidson_opts* o) {
   o->verbose = 0;
   o->energy_ref = 0.0;
   o->energy_tol = 1.0e-6;
+  o->method = 0;
+  o->block = 0;
+  o->stop_at_tol = 0;
 }
 
-static void check_opts(const sbdx_davidson_opts* o) {
+// validates the options; returns the vectors per block of the device work buffer
+static int check_opts(const sbdx_ctx* c, const sbdx_davidson_opts* o) {
   need(o != nullptr, "null options");
   need(o->max_iteration >= 1 && o->num_block >= 1 && o->num_block <= 64, "bad iteration/block");
-  need(o->nroots == 1, "nroots > 1 is not available in this build");
-  need(o->precision == SBDX_FP64, "only FP64 Davidson is available in this build");
+  need(o->nroots >= 1 && o->nroots <= 8, "nroots must be 1..8");
+  need(o->precision == SBDX_FP64 || o->precision == SBDX_FP32, "precision must be SBDX_FP64 or SBDX_FP32");
+  need(o->preconditioner == 0, "preconditioner must be 0 (diagonal)");
+  need(o->method == 0 || o->method == 1, "method must be 0 or 1");
+  need(o->block >= 0 && o->block <= 8, "block must be 0..8");
+  if (o->method == 0) {
+    need(o->nroots == 1 && o->block <= 1 && o->precision == SBDX_FP64 && o->preconditioner == 0,
+         "method 0 (the reference algorithm) is FP64, one root, diagonal preconditioner; use method 1");
+    return 1;
+  }
+  const int k = sbdx::davidson_block_width(*o);
+  need(c->ndet() >= k, "fewer determinants than vectors per block");
+  return k;
+}
+
+static void run_davidson(sbdx_ctx& c, const sbdx_davidson_opts& o, double* w, sbdx_davidson_result& res,
+                         double* trace, double* resid, int trace_cap) {
+  if (o.method == 0) sbdx::davidson_fp64(c, o, w, res, trace, resid, trace_cap);
+  else sbdx::davidson_block(c, o, w, res, trace, resid, trace_cap);
 }
 
 int sbdx_davidson_device(sbdx_ctx* c, const sbdx_davidson_opts* o, void* dW, sbdx_davidson_result* res,
                          double* trace, double* resid, int trace_cap) {
   return guarded([&] {
     need_ints(c);
-    check_opts(o);
+    const int k = check_opts(c, o);
     need(dW && res, "null argument");
     std::memset(res, 0, sizeof(*res));
     const int64_t n = c->ndet();
-    sbdx::DevBuf<double> w(n);
+    double* user = static_cast<double*>(dW);
+    sbdx::DevBuf<double> w(n * k);
     if (o->init == 0) sbdx::init_hf(*c, w.get(), c->stream);
-    else sbdx::apply_sign(*c, SBDX_FP64, 1, dW, w.get(), c->stream);
-    sbdx::davidson_fp64(*c, *o, w.get(), *res, trace, resid, trace_cap);
-    sbdx::apply_sign(*c, SBDX_FP64, 1, w.get(), dW, c->stream);
+    else sbdx::apply_sign(*c, SBDX_FP64, 1, user, w.get(), c->stream);
+    run_davidson(*c, *o, w.get(), *res, trace, resid, trace_cap);
+    for (int v = 0; v < o->nroots; ++v) sbdx::apply_sign(*c, SBDX_FP64, 1, w.get() + v * n, user + v * n, c->stream);
     SBDX_HIP(hipStreamSynchronize(c->stream));
   });
 }
@@ -232,15 +274,15 @@ int sbdx_davidson(sbdx_ctx* c, const sbdx_davidson_opts* o, double* W, sbdx_davi
                   double* trace, double* resid, int trace_cap) {
   return guarded([&] {
     need_ints(c);
-    check_opts(o);
+    const int k = check_opts(c, o);
     need(W && res, "null argument");
     std::memset(res, 0, sizeof(*res));
     const int64_t n = c->ndet();
-    sbdx::DevBuf<double> w(n), tmp;
+    sbdx::DevBuf<double> w(n * k), tmp;
     if (o->init == 0) sbdx::init_hf(*c, w.get(), c->stream);
     else upload_blocked(*c, W, w.get(), tmp);
-    sbdx::davidson_fp64(*c, *o, w.get(), *res, trace, resid, trace_cap);
-    download_user(*c, w.get(), W, tmp);
+    run_davidson(*c, *o, w.get(), *res, trace, resid, trace_cap);
+    for (int v = 0; v < o->nroots; ++v) download_user(*c, w.get() + v * n, W + v * n, tmp);
   });
 }
 

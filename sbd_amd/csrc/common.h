@@ -119,7 +119,7 @@ struct sbdx_ctx {
   int L = 0;
   int64_t na = 0, nb = 0;
   sbdx::SpinTab A, B;
-  sbdx::MixTiles mix;
+  sbdx::MixTiles mix[4];           // mixed-pass tiles for 4, 8, 16, 32 entries per thread
   bool have_ints = false;
   double I0 = 0.0;
   std::vector<double> h_I1, h_I2;
@@ -200,8 +200,12 @@ __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta);  // tables.hip
 void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
 void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
-void build_mix_tiles(sbdx_ctx& c);              // sigma.hip: mixed-pass tiles
+void reset_mix_tiles(sbdx_ctx& c);              // sigma.hip: drop cached mixed-pass tiles
+// sigma of nvec vectors stored one after another (vector-major)
 void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
+// fused sigma of a block of k vectors stored interleaved, element [det * k + v], k in {1,2,4,8}
+void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
 void apply_sign(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
+void apply_sign_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
 
 }  // namespace sbdx

@@ -351,6 +351,8 @@ void davidson_fp64(sbdx_ctx& c, const sbdx_davidson_opts& o, double* dW, sbdx_da
   (void)e_last;
 }
 
+void jacobi_syev_host(int n, double* a, int lda, double* w) { jacobi_syev(n, a, lda, w); }
+
 void init_hf(sbdx_ctx& c, double* dW, hipStream_t st) {
   // HF vector W[0] = 1 (davidson.h:101-140, init 0) expressed in the blocked basis
   uint64_t a0 = 0, b0 = 0;

@@ -1,11 +1,14 @@
 // sigma.hip -- the on-the-fly H C product (chemistry/tpb/mult.h:387-648) in the blocked basis.
 //
-// sigma = D C + H_aa C + H_bb C + H_ab C, with C an n_alpha x n_beta row-major matrix.
+// sigma = D C + H_aa C + H_bb C + H_ab C, with C an n_alpha x n_beta row-major matrix, for a
+// block of K vectors stored interleaved (element [det * K + v]); K = 1 is the single-vector
+// case.  One gather then loads K contiguous values, and list decoding, integral lookups and
+// matrix-element arithmetic are shared by the K vectors.
 //
 //  * Row pass (alpha rows, task type 2 + diagonal): one workgroup per (alpha row ia, tile of
 //    256 beta columns); every wavefront covers 64 consecutive columns of the same row, so the
 //    row's excitation lists are wavefront-uniform (scalar loads) and every gathered operand
-//    C[ja, ib..ib+63] is one coalesced 512-byte (FP64) segment.
+//    C[ja, ib..ib+63] is one coalesced segment (512 K bytes in FP64).
 //  * Mixed pass (task type 0): k_sigma_mixed, described at the kernel.
 //  * Beta part (task type 1): C is transposed, the same row kernel runs with the beta tables,
 //    and the result is added back transposed.
@@ -15,8 +18,8 @@
 //   alpha single ja -> ia (i -> a):  s_a (F_a + sum_{j in beta(ib)} (ai|jj))
 //   alpha double:                     stored value (sign and exchange included)
 //   alpha single x beta single:       s_a s_b (ai|bj)
-// Arithmetic: FP64 (plain) or FP32 with compensated accumulation (error-free product and
-// sum transformations, i.e. Ogita-Rump-Oishi Dot2 per output element).
+// Arithmetic: FP64 (FMA) or FP32 with compensated accumulation (error-free product and sum
+// transformations, i.e. Ogita-Rump-Oishi Dot2 per output element).
 #include "common.h"
 
 #include <algorithm>
@@ -27,7 +30,6 @@ namespace sbdx {
 namespace {
 
 constexpr int kTile = 256;
-constexpr int kUnroll = 8;  // gathers issued back to back per wavefront in the row kernel
 
 struct AccD {
   double s = 0.0;
@@ -57,6 +59,22 @@ template <typename T> struct AccOf;
 template <> struct AccOf<double> { using type = AccD; };
 template <> struct AccOf<float> { using type = AccF; };
 
+// K contiguous values, loaded and stored as whole 16-byte words where possible
+template <typename T, int K>
+struct alignas(K * sizeof(T) >= 16 ? 16 : K * sizeof(T)) VecK {
+  T v[K];
+};
+
+template <typename T, int K>
+__device__ inline VecK<T, K> ldk(const T* p) {
+  return *reinterpret_cast<const VecK<T, K>*>(p);
+}
+
+template <typename T, int K>
+__device__ inline void stk(T* p, const VecK<T, K>& a) {
+  *reinterpret_cast<VecK<T, K>*>(p) = a;
+}
+
 struct RowTabs {
   const int64_t* off_s;
   const uint32_t* ket_s;
@@ -66,34 +84,51 @@ struct RowTabs {
 };
 
 // Same-spin part (and, for the alpha rows, the diagonal): y[row, col] = sum over the row's
-// singles and doubles of value * x[ket, col].
-template <typename T, bool kDiag>
+// singles and doubles of value * x[ket, col], for K interleaved vectors.
+template <typename T, bool kDiag, int K>
 __global__ __launch_bounds__(kTile) void k_sigma_rows(
     int64_t nrow, int64_t ncol, int L, RowTabs rt, const T* __restrict__ val_s,
     const T* __restrict__ val_d, const uint64_t* __restrict__ col_str, const T* __restrict__ V,
     const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ hii) {
+  // gathers issued back to back per wavefront before their FMAs
+  constexpr int U = K == 1 ? 8 : (K == 2 ? 4 : 2);
   const int64_t g = blockIdx.x;
   const int64_t tile = g / nrow;
   const int64_t row = g - tile * nrow;
   const int64_t col = tile * kTile + threadIdx.x;
   if (col >= ncol) return;
   const int64_t L2 = static_cast<int64_t>(L) * L;
-  typename AccOf<T>::type acc;
+  typename AccOf<T>::type acc[K];
   const int64_t me = row * ncol + col;
-  if (kDiag) acc.fma(hii[me], x[me]);
-
-  // same-spin doubles, kUnroll gathers in flight per wavefront
-  const int64_t d0 = rt.off_d[row], d1 = rt.off_d[row + 1];
-  const T* __restrict__ xc = x + col;
-  int64_t k = d0;
-  for (; k + kUnroll <= d1; k += kUnroll) {
-    T xv[kUnroll];
+  if (kDiag) {
+    const VecK<T, K> xm = ldk<T, K>(x + me * K);
+    const T h = hii[me];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) xv[u] = xc[static_cast<int64_t>(rt.ket_d[k + u]) * ncol];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) acc.fma(val_d[k + u], xv[u]);
+    for (int v = 0; v < K; ++v) acc[v].fma(h, xm.v[v]);
   }
-  for (; k < d1; ++k) acc.fma(val_d[k], xc[static_cast<int64_t>(rt.ket_d[k]) * ncol]);
+
+  // same-spin doubles
+  const int64_t d0 = rt.off_d[row], d1 = rt.off_d[row + 1];
+  const T* __restrict__ xc = x + col * K;
+  const int64_t stride = ncol * K;
+  int64_t k = d0;
+  for (; k + U <= d1; k += U) {
+    VecK<T, K> xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_d[k + u]) * stride);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T w = val_d[k + u];
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[v].fma(w, xv[u].v[v]);
+    }
+  }
+  for (; k < d1; ++k) {
+    const VecK<T, K> xv = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_d[k]) * stride);
+    const T w = val_d[k];
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v].fma(w, xv.v[v]);
+  }
 
   // same-spin singles, with the spectator string's Coulomb term sum_{j in col string} (ai|jj)
   const uint64_t ycol = col_str[col];
@@ -111,15 +146,27 @@ __global__ __launch_bounds__(kTile) void k_sigma_rows(
     return val_s[q] + ((e >> 12) & 1u ? -coul : coul);
   };
   k = s0;
-  for (; k + kUnroll <= s1; k += kUnroll) {
-    T xv[kUnroll];
+  for (; k + U <= s1; k += U) {
+    VecK<T, K> xv[U];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) xv[u] = xc[static_cast<int64_t>(rt.ket_s[k + u]) * ncol];
+    for (int u = 0; u < U; ++u) xv[u] = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_s[k + u]) * stride);
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) acc.fma(single_value(k + u), xv[u]);
+    for (int u = 0; u < U; ++u) {
+      const T w = single_value(k + u);
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[v].fma(w, xv[u].v[v]);
+    }
   }
-  for (; k < s1; ++k) acc.fma(single_value(k), xc[static_cast<int64_t>(rt.ket_s[k]) * ncol]);
-  y[me] = acc.get();
+  for (; k < s1; ++k) {
+    const VecK<T, K> xv = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_s[k]) * stride);
+    const T w = single_value(k);
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v].fma(w, xv.v[v]);
+  }
+  VecK<T, K> out;
+#pragma unroll
+  for (int v = 0; v < K; ++v) out.v[v] = acc[v].get();
+  stk<T, K>(y + me * K, out);
 }
 
 // Alpha single x beta single (task type 0, mult.h:569-596):
@@ -132,11 +179,10 @@ __global__ __launch_bounds__(kTile) void k_sigma_rows(
 // share that row's ket rows in L2.  Per-entry sums are folded into the tile's columns with LDS
 // atomics once at the end; sigma is updated by its owner, with no global atomics.
 constexpr int kMixThreads = 256;
-constexpr int kMixEPTDefault = 16;                // entries per thread (8, 16 or 32)
 constexpr int kMixMaxCols = 1024;
 // V row elements fetched per thread: 8 covers L^2 <= 2048 (L <= 45), 16 covers L <= 64
 
-template <typename T, int kMixEPT, int kVPerThread>
+template <typename T, int kEPT, int kVPerThread, int K>
 __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     int64_t na, int64_t nb, int L, const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket,
     const uint32_t* __restrict__ a_exc, int ntile, const int64_t* __restrict__ t_off,
@@ -162,19 +208,19 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
 
   const int64_t e0 = t_off[tile], e1 = t_off[tile + 1];
   const int col0 = t_col0[tile], ncol = t_ncol[tile];
-  uint32_t jb[kMixEPT], meta[kMixEPT];
-  typename AccOf<T>::type acc[kMixEPT];
+  uint32_t jb[kEPT], meta[kEPT];
+  typename AccOf<T>::type acc[kEPT][K];
   // padding entries (past the tile's end) read column 0 and pair 0, which are always valid, so
   // the entry loop below has no branches and all its gathers can be in flight at once; their
   // sums are dropped at the end (ib_local 0xffff)
 #pragma unroll
-  for (int j = 0; j < kMixEPT; ++j) {
+  for (int j = 0; j < kEPT; ++j) {
     const int64_t e = e0 + tid + static_cast<int64_t>(j) * kMixThreads;
     const bool ok = e < e1;
     jb[j] = ok ? e_jb[e] : 0u;
     meta[j] = ok ? e_meta[e] : 0x0000ffffu;
   }
-  for (int c = tid; c < ncol; c += kMixThreads) out[c] = T(0);
+  for (int c = tid; c < ncol * K; c += kMixThreads) out[c] = T(0);
 
   // V row of the next alpha single: fetched into registers before the current row's gathers
   // (its latency hides under them), written to LDS after them
@@ -205,67 +251,83 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     __syncthreads();  // cur is staged; nobody reads nxt any more
     const bool more = k + 1 < k1;
     if (more) fetch(k + 1);
-    const T* __restrict__ xr = x + static_cast<int64_t>(a_ket[k]) * nb;
-    T xv[kMixEPT];
+    const T* __restrict__ xr = x + static_cast<int64_t>(a_ket[k]) * nb * K;
+    VecK<T, K> xv[kEPT];
 #pragma unroll
-    for (int j = 0; j < kMixEPT; ++j) xv[j] = xr[jb[j]];
+    for (int j = 0; j < kEPT; ++j) xv[j] = ldk<T, K>(xr + static_cast<int64_t>(jb[j]) * K);
 #pragma unroll
-    for (int j = 0; j < kMixEPT; ++j) acc[j].fma(cur[(meta[j] >> 16) & 0xfff], xv[j]);
+    for (int j = 0; j < kEPT; ++j) {
+      const T w = cur[(meta[j] >> 16) & 0xfff];
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[j][v].fma(w, xv[j].v[v]);
+    }
     if (more) commit(nxt);
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < kMixEPT; ++j) {
+  for (int j = 0; j < kEPT; ++j) {
     const uint32_t m = meta[j];
     if ((m & 0xffffu) != 0xffffu) {
-      const T v = acc[j].get();
-      atomicAdd(&out[m & 0xffffu], (m >> 31) ? -v : v);
+#pragma unroll
+      for (int v = 0; v < K; ++v) {
+        const T s = acc[j][v].get();
+        atomicAdd(&out[(m & 0xffffu) * K + v], (m >> 31) ? -s : s);
+      }
     }
   }
   __syncthreads();
-  T* yr = y + ia * nb + col0;
-  for (int c = tid; c < ncol; c += kMixThreads) yr[c] += out[c];
+  T* yr = y + (ia * nb + col0) * K;
+  for (int c = tid; c < ncol * K; c += kMixThreads) yr[c] += out[c];
 }
 
-// out[c * R + r] = in[r * C + c]
-template <typename T>
+// out[c * R + r] = in[r * C + c] on K-value units; S x S tiles staged in LDS
+template <typename T, int K>
 __global__ __launch_bounds__(256) void k_transpose(const T* __restrict__ in, T* __restrict__ out,
                                                    int64_t R, int64_t C) {
-  __shared__ T tile[64][65];
-  const int64_t tc = cdiv(C, 64);
+  constexpr int S = K * sizeof(T) <= 8 ? 64 : (K * sizeof(T) <= 32 ? 32 : 16);
+  __shared__ VecK<T, K> tile[S][S + 1];
+  const int64_t tc = cdiv(C, S);
   const int64_t br = blockIdx.x / tc, bc = blockIdx.x % tc;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int k = ty; k < 64; k += 4) {
-    const int64_t r = br * 64 + k, c = bc * 64 + tx;
-    if (r < R && c < C) tile[k][tx] = in[r * C + c];
+  const int tx = threadIdx.x % S, ty = threadIdx.x / S;
+  constexpr int STEP = 256 / S;
+  for (int k = ty; k < S; k += STEP) {
+    const int64_t r = br * S + k, c = bc * S + tx;
+    if (r < R && c < C) tile[k][tx] = ldk<T, K>(in + (r * C + c) * K);
   }
   __syncthreads();
-  for (int k = ty; k < 64; k += 4) {
-    const int64_t c = bc * 64 + k, r = br * 64 + tx;
-    if (r < R && c < C) out[c * R + r] = tile[tx][k];
+  for (int k = ty; k < S; k += STEP) {
+    const int64_t c = bc * S + k, r = br * S + tx;
+    if (r < R && c < C) stk<T, K>(out + (c * R + r) * K, tile[tx][k]);
   }
 }
 
-// y[r * C + c] += inT[c * R + r]
-template <typename T>
+// y[r * C + c] += inT[c * R + r] on K-value units
+template <typename T, int K>
 __global__ __launch_bounds__(256) void k_add_transposed(const T* __restrict__ inT, T* __restrict__ y,
                                                         int64_t R, int64_t C) {
-  __shared__ T tile[64][65];
-  const int64_t tr = cdiv(R, 64);
+  constexpr int S = K * sizeof(T) <= 8 ? 64 : (K * sizeof(T) <= 32 ? 32 : 16);
+  __shared__ VecK<T, K> tile[S][S + 1];
+  const int64_t tr = cdiv(R, S);
   const int64_t bc = blockIdx.x / tr, br = blockIdx.x % tr;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int k = ty; k < 64; k += 4) {
-    const int64_t c = bc * 64 + k, r = br * 64 + tx;
-    if (r < R && c < C) tile[k][tx] = inT[c * R + r];
+  const int tx = threadIdx.x % S, ty = threadIdx.x / S;
+  constexpr int STEP = 256 / S;
+  for (int k = ty; k < S; k += STEP) {
+    const int64_t c = bc * S + k, r = br * S + tx;
+    if (r < R && c < C) tile[k][tx] = ldk<T, K>(inT + (c * R + r) * K);
   }
   __syncthreads();
-  for (int k = ty; k < 64; k += 4) {
-    const int64_t r = br * 64 + k, c = bc * 64 + tx;
-    if (r < R && c < C) y[r * C + c] += tile[tx][k];
+  for (int k = ty; k < S; k += STEP) {
+    const int64_t r = br * S + k, c = bc * S + tx;
+    if (r < R && c < C) {
+      VecK<T, K> a = ldk<T, K>(y + (r * C + c) * K);
+#pragma unroll
+      for (int v = 0; v < K; ++v) a.v[v] += tile[tx][k].v[v];
+      stk<T, K>(y + (r * C + c) * K, a);
+    }
   }
 }
 
-template <typename T>
+template <typename T, int K>
 __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__ sa,
                                                     const uint64_t* __restrict__ sb, int64_t na,
                                                     int64_t nb, const T* __restrict__ x,
@@ -274,97 +336,42 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
   const int64_t ia = blockIdx.x / nt;
   const int64_t ib = (blockIdx.x % nt) * 256 + threadIdx.x;
   if (ib >= nb) return;
-  const int s = det_sign(sa[ia], sb[ib]);
-  const int64_t k = ia * nb + ib;
-  y[k] = s < 0 ? -x[k] : x[k];
+  const bool neg = det_sign(sa[ia], sb[ib]) < 0;
+  const int64_t k = (ia * nb + ib) * K;
+  VecK<T, K> a = ldk<T, K>(x + k);
+#pragma unroll
+  for (int v = 0; v < K; ++v) a.v[v] = neg ? -a.v[v] : a.v[v];
+  stk<T, K>(y + k, a);
 }
 
-template <typename T>
-void sigma_one(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const T* vd_a,
-               const T* vs_b, const T* vd_b, const T* V, const T* hii) {
-  const int64_t na = c.na, nb = c.nb;
-  const int64_t n = na * nb;
-  RowTabs ta{c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), c.A.off_d.get(), c.A.ket_d.get()};
-  RowTabs tb{c.B.off_s.get(), c.B.ket_s.get(), c.B.exc_s.get(), c.B.off_d.get(), c.B.ket_d.get()};
-  Prof& pf = c.prof;
-  auto mark = [&](int k) {
-    if (pf.on) SBDX_HIP(hipEventRecord(pf.ev[k], st));
-  };
-  mark(0);
-  // alpha rows: diagonal + alpha singles and doubles
-  {
-    const int64_t blocks = na * cdiv(nb, kTile);
-    k_sigma_rows<T, true><<<static_cast<unsigned>(blocks), kTile, 0, st>>>(
-        na, nb, c.L, ta, vs_a, vd_a, c.B.str.get(), V, x, y, hii);
-    SBDX_CHECK_LAUNCH();
+// entries per thread of the mixed kernel for a block width K (register budget of the
+// per-entry accumulators); K = 1 may be overridden by SBDX_MIX_EPT
+int mix_ept(int K) {
+  if (K == 1) {
+    const char* env = std::getenv("SBDX_MIX_EPT");
+    const int want = env ? std::atoi(env) : 16;
+    return (want == 8 || want == 16 || want == 32) ? want : 16;
   }
-  mark(1);
-  // alpha single x beta single
-  if (c.mix.ntile > 0) {
-    const MixTiles& mt = c.mix;
-    const int64_t blocks = 8 * cdiv(na, 8) * mt.ntile;
-    const size_t lds = (2 * static_cast<size_t>(c.L) * c.L + kMixMaxCols) * sizeof(T);
-    auto launch = [&](auto kernel) {
-      if (lds > 65536)
-        SBDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-      kernel<<<static_cast<unsigned>(blocks), kMixThreads, lds, st>>>(
-          na, nb, c.L, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
-          mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), V, x, y);
-      SBDX_CHECK_LAUNCH();
-    };
-    const bool small_v = static_cast<int>(c.L) * c.L <= 8 * kMixThreads;
-    if (mt.ept == 8) small_v ? launch(&k_sigma_mixed<T, 8, 8>) : launch(&k_sigma_mixed<T, 8, 16>);
-    else if (mt.ept == 16) small_v ? launch(&k_sigma_mixed<T, 16, 8>) : launch(&k_sigma_mixed<T, 16, 16>);
-    else small_v ? launch(&k_sigma_mixed<T, 32, 8>) : launch(&k_sigma_mixed<T, 32, 16>);
-  }
-  mark(2);
-  // beta rows on the transposed vector
-  c.ws.xT.ensure(n * sizeof(T));
-  c.ws.yT.ensure(n * sizeof(T));
-  T* xT = reinterpret_cast<T*>(c.ws.xT.get());
-  T* yT = reinterpret_cast<T*>(c.ws.yT.get());
-  const unsigned tblocks = static_cast<unsigned>(cdiv(na, 64) * cdiv(nb, 64));
-  k_transpose<T><<<tblocks, 256, 0, st>>>(x, xT, na, nb);
-  SBDX_CHECK_LAUNCH();
-  mark(3);
-  {
-    const int64_t blocks = nb * cdiv(na, kTile);
-    k_sigma_rows<T, false><<<static_cast<unsigned>(blocks), kTile, 0, st>>>(
-        nb, na, c.L, tb, vs_b, vd_b, c.A.str.get(), V, xT, yT, nullptr);
-    SBDX_CHECK_LAUNCH();
-  }
-  mark(4);
-  k_add_transposed<T><<<tblocks, 256, 0, st>>>(yT, y, na, nb);
-  SBDX_CHECK_LAUNCH();
-  mark(5);
-  if (pf.on) {
-    SBDX_HIP(hipEventSynchronize(pf.ev[kPhases]));
-    for (int k = 0; k < kPhases; ++k) {
-      float ms = 0.0f;
-      SBDX_HIP(hipEventElapsedTime(&ms, pf.ev[k], pf.ev[k + 1]));
-      pf.ms[k] += ms;
-    }
-    pf.calls++;
-  }
+  return K == 2 ? 8 : 4;
 }
 
-}  // namespace
+int ept_slot(int ept) { return ept == 4 ? 0 : ept == 8 ? 1 : ept == 16 ? 2 : 3; }
 
-// Regroup the beta singles into mixed-pass tiles (host side, once per integral set: the packed
-// excitation words carry the orbital pair and sign).
-void build_mix_tiles(sbdx_ctx& c) {
+// Regroup the beta singles into mixed-pass tiles of at most 256 * ept entries (host side, once
+// per integral set and ept: the packed excitation words carry the orbital pair and sign).
+const MixTiles& mix_tiles(sbdx_ctx& c, int ept) {
+  MixTiles& mt = c.mix[ept_slot(ept)];
+  if (mt.ept == ept && mt.ent_off.get()) return mt;
+  mt = MixTiles{};
+  mt.ept = ept;
   const SpinTab& B = c.B;
   const int L = c.L;
   const int64_t nb = B.n;
-  MixTiles& mt = c.mix;
-  mt = MixTiles{};
-  // entries per thread: a tuning knob (register footprint vs. re-reads of ket rows per tile)
-  const char* env = std::getenv("SBDX_MIX_EPT");
-  const int want = env ? std::atoi(env) : kMixEPTDefault;
-  mt.ept = (want == 8 || want == 16) ? want : 32;
-  const int64_t kMixCap = static_cast<int64_t>(kMixThreads) * mt.ept;
-  if (B.nnz_s == 0 || c.A.nnz_s == 0) return;
+  const int64_t cap = static_cast<int64_t>(kMixThreads) * ept;
+  if (B.nnz_s == 0 || c.A.nnz_s == 0) {
+    mt.ent_off.alloc(1);
+    return mt;
+  }
   std::vector<int64_t> off(nb + 1);
   std::vector<uint32_t> ket(B.nnz_s), exc(B.nnz_s);
   SBDX_HIP(hipMemcpy(off.data(), B.off_s.get(), (nb + 1) * 8, hipMemcpyDeviceToHost));
@@ -380,7 +387,7 @@ void build_mix_tiles(sbdx_ctx& c) {
   while (ib < nb) {
     const int64_t start = ib;
     int64_t cnt = 0;
-    while (ib < nb && ib - start < kMixMaxCols && cnt + (off[ib + 1] - off[ib]) <= kMixCap) {
+    while (ib < nb && ib - start < kMixMaxCols && cnt + (off[ib + 1] - off[ib]) <= cap) {
       cnt += off[ib + 1] - off[ib];
       ++ib;
     }
@@ -414,39 +421,146 @@ void build_mix_tiles(sbdx_ctx& c) {
   SBDX_HIP(hipMemcpy(mt.ncol.get(), t_ncol.data(), t_ncol.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.jb.get(), e_jb.data(), e_jb.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.meta.get(), e_meta.data(), e_meta.size() * 4, hipMemcpyHostToDevice));
+  return mt;
+}
+
+template <typename T, int K>
+void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
+  const int ept = mix_ept(K);
+  const MixTiles& mt = mix_tiles(c, ept);
+  if (mt.ntile == 0) return;
+  const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
+  const size_t lds = (2 * static_cast<size_t>(c.L) * c.L + static_cast<size_t>(mt.max_cols) * K) * sizeof(T);
+  auto launch = [&](auto kernel) {
+    if (lds > 65536)
+      SBDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    kernel<<<static_cast<unsigned>(blocks), kMixThreads, lds, st>>>(
+        c.na, c.nb, c.L, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
+        mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), V, x, y);
+    SBDX_CHECK_LAUNCH();
+  };
+  const bool small_v = static_cast<int>(c.L) * c.L <= 8 * kMixThreads;
+  if constexpr (K == 1) {
+    if (ept == 8) small_v ? launch(&k_sigma_mixed<T, 8, 8, 1>) : launch(&k_sigma_mixed<T, 8, 16, 1>);
+    else if (ept == 16) small_v ? launch(&k_sigma_mixed<T, 16, 8, 1>) : launch(&k_sigma_mixed<T, 16, 16, 1>);
+    else small_v ? launch(&k_sigma_mixed<T, 32, 8, 1>) : launch(&k_sigma_mixed<T, 32, 16, 1>);
+  } else if constexpr (K == 2) {
+    small_v ? launch(&k_sigma_mixed<T, 8, 8, 2>) : launch(&k_sigma_mixed<T, 8, 16, 2>);
+  } else {
+    small_v ? launch(&k_sigma_mixed<T, 4, 8, K>) : launch(&k_sigma_mixed<T, 4, 16, K>);
+  }
+}
+
+template <typename T, int K>
+void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const T* vd_a,
+             const T* vs_b, const T* vd_b, const T* V, const T* hii) {
+  const int64_t na = c.na, nb = c.nb;
+  const int64_t n = na * nb;
+  RowTabs ta{c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), c.A.off_d.get(), c.A.ket_d.get()};
+  RowTabs tb{c.B.off_s.get(), c.B.ket_s.get(), c.B.exc_s.get(), c.B.off_d.get(), c.B.ket_d.get()};
+  Prof& pf = c.prof;
+  auto mark = [&](int k) {
+    if (pf.on) SBDX_HIP(hipEventRecord(pf.ev[k], st));
+  };
+  mark(0);
+  // alpha rows: diagonal + alpha singles and doubles
+  k_sigma_rows<T, true, K><<<static_cast<unsigned>(na * cdiv(nb, kTile)), kTile, 0, st>>>(
+      na, nb, c.L, ta, vs_a, vd_a, c.B.str.get(), V, x, y, hii);
+  SBDX_CHECK_LAUNCH();
+  mark(1);
+  // alpha single x beta single
+  launch_mixed<T, K>(c, x, y, V, st);
+  mark(2);
+  // beta rows on the transposed vector
+  c.ws.xT.ensure(n * K * sizeof(T));
+  c.ws.yT.ensure(n * K * sizeof(T));
+  T* xT = reinterpret_cast<T*>(c.ws.xT.get());
+  T* yT = reinterpret_cast<T*>(c.ws.yT.get());
+  constexpr int S = K * sizeof(T) <= 8 ? 64 : (K * sizeof(T) <= 32 ? 32 : 16);
+  const unsigned tblocks = static_cast<unsigned>(cdiv(na, S) * cdiv(nb, S));
+  k_transpose<T, K><<<tblocks, 256, 0, st>>>(x, xT, na, nb);
+  SBDX_CHECK_LAUNCH();
+  mark(3);
+  k_sigma_rows<T, false, K><<<static_cast<unsigned>(nb * cdiv(na, kTile)), kTile, 0, st>>>(
+      nb, na, c.L, tb, vs_b, vd_b, c.A.str.get(), V, xT, yT, nullptr);
+  SBDX_CHECK_LAUNCH();
+  mark(4);
+  k_add_transposed<T, K><<<tblocks, 256, 0, st>>>(yT, y, na, nb);
+  SBDX_CHECK_LAUNCH();
+  mark(5);
+  if (pf.on) {
+    SBDX_HIP(hipEventSynchronize(pf.ev[kPhases]));
+    for (int k = 0; k < kPhases; ++k) {
+      float ms = 0.0f;
+      SBDX_HIP(hipEventElapsedTime(&ms, pf.ev[k], pf.ev[k + 1]));
+      pf.ms[k] += ms;
+    }
+    pf.calls++;
+  }
+}
+
+template <typename T>
+void sigma_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
+  const bool f = std::is_same<T, float>::value;
+  if (f) ensure_fp32(c);
+  const T* vs_a = f ? reinterpret_cast<const T*>(c.A.val_s32.get()) : reinterpret_cast<const T*>(c.A.val_s.get());
+  const T* vd_a = f ? reinterpret_cast<const T*>(c.A.val_d32.get()) : reinterpret_cast<const T*>(c.A.val_d.get());
+  const T* vs_b = f ? reinterpret_cast<const T*>(c.B.val_s32.get()) : reinterpret_cast<const T*>(c.B.val_s.get());
+  const T* vd_b = f ? reinterpret_cast<const T*>(c.B.val_d32.get()) : reinterpret_cast<const T*>(c.B.val_d.get());
+  const T* V = f ? reinterpret_cast<const T*>(c.eri4_32.get()) : reinterpret_cast<const T*>(c.eri4.get());
+  const T* hii = f ? reinterpret_cast<const T*>(c.hii32.get()) : reinterpret_cast<const T*>(c.hii.get());
+  switch (k) {
+    case 1: sigma_k<T, 1>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+    case 2: sigma_k<T, 2>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+    case 4: sigma_k<T, 4>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+    case 8: sigma_k<T, 8>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+    default: throw Error(-1, "block width must be 1, 2, 4 or 8");
+  }
+}
+
+template <typename T>
+void sign_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
+  const unsigned blocks = static_cast<unsigned>(c.na * cdiv(c.nb, 256));
+  switch (k) {
+    case 1: k_apply_sign<T, 1><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
+    case 2: k_apply_sign<T, 2><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
+    case 4: k_apply_sign<T, 4><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
+    case 8: k_apply_sign<T, 8><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
+    default: throw Error(-1, "block width must be 1, 2, 4 or 8");
+  }
+  SBDX_CHECK_LAUNCH();
+}
+
+}  // namespace
+
+void reset_mix_tiles(sbdx_ctx& c) {
+  for (auto& m : c.mix) m = MixTiles{};
+}
+
+void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {
+  if (!c.have_ints) throw Error(-2, "integrals not set");
+  if (prec == 0) sigma_dispatch<double>(c, k, static_cast<const double*>(x), static_cast<double*>(y), st);
+  else sigma_dispatch<float>(c, k, static_cast<const float*>(x), static_cast<float*>(y), st);
 }
 
 void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t st) {
-  if (!c.have_ints) throw Error(-2, "integrals not set");
   const int64_t n = c.ndet();
-  for (int v = 0; v < nvec; ++v) {
-    if (prec == 0) {
-      sigma_one<double>(c, static_cast<const double*>(x) + v * n, static_cast<double*>(y) + v * n, st,
-                        c.A.val_s.get(), c.A.val_d.get(), c.B.val_s.get(), c.B.val_d.get(),
-                        c.eri4.get(), c.hii.get());
-    } else {
-      ensure_fp32(c);
-      sigma_one<float>(c, static_cast<const float*>(x) + v * n, static_cast<float*>(y) + v * n, st,
-                       c.A.val_s32.get(), c.A.val_d32.get(), c.B.val_s32.get(), c.B.val_d32.get(),
-                       c.eri4_32.get(), c.hii32.get());
-    }
-  }
+  const size_t es = prec == 0 ? 8 : 4;
+  for (int v = 0; v < nvec; ++v)
+    sigma_block(c, prec, 1, static_cast<const char*>(x) + v * n * es, static_cast<char*>(y) + v * n * es, st);
+}
+
+void apply_sign_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {
+  if (prec == 0) sign_dispatch<double>(c, k, static_cast<const double*>(x), static_cast<double*>(y), st);
+  else sign_dispatch<float>(c, k, static_cast<const float*>(x), static_cast<float*>(y), st);
 }
 
 void apply_sign(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t st) {
   const int64_t n = c.ndet();
-  const unsigned blocks = static_cast<unsigned>(c.na * cdiv(c.nb, 256));
-  for (int v = 0; v < nvec; ++v) {
-    if (prec == 0)
-      k_apply_sign<double><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb,
-                                                   static_cast<const double*>(x) + v * n,
-                                                   static_cast<double*>(y) + v * n);
-    else
-      k_apply_sign<float><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb,
-                                                  static_cast<const float*>(x) + v * n,
-                                                  static_cast<float*>(y) + v * n);
-    SBDX_CHECK_LAUNCH();
-  }
+  const size_t es = prec == 0 ? 8 : 4;
+  for (int v = 0; v < nvec; ++v)
+    apply_sign_block(c, prec, 1, static_cast<const char*>(x) + v * n * es, static_cast<char*>(y) + v * n * es, st);
 }
 
 }  // namespace sbdx

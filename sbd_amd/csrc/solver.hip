@@ -1,0 +1,663 @@
+// solver.hip -- block thick-restart Davidson with mixed precision (method 1).
+//
+// Subspace: the anchor block X (K vectors, FP64) plus up to m-1 correction blocks stored in
+// precision P (FP64, or FP32 for the mixed-precision solver).  Every restart recomputes the
+// anchor's sigma in FP64, so the residual that decides convergence is exact FP64; the correction
+// blocks and their sigmas carry P rounding, which enters the Ritz vectors only through their
+// coefficients, which vanish as the iteration converges (an outer FP64 refinement loop).
+// Rayleigh-Ritz solves H_sub u = e S_sub u with both matrices accumulated in FP64 from the stored
+// blocks, by canonical orthogonalization, so FP32 blocks need not be exactly orthonormal.  New
+// blocks: preconditioned residuals, two passes of classical Gram-Schmidt against the subspace in
+// FP64 arithmetic, then Loewdin orthonormalization of the block itself.  Thick restart keeps the
+// K Ritz vectors of the whole subspace as the next anchor.
+//
+// Every block product is tall-skinny: the kernels stream their blocks once ([N][K] interleaved,
+// K contiguous values per determinant) and reduce K x K partial sums per workgroup.
+//
+// The single-vector solver that follows the reference step for step (chemistry/tpb/davidson.h:
+// 661-923) is davidson_fp64 (davidson.hip, method 0).
+#include "common.h"
+#include "sbdx.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <type_traits>
+
+namespace sbdx {
+
+void jacobi_syev_host(int n, double* a, int lda, double* w);  // davidson.hip
+
+namespace {
+
+constexpr int kB = 256;
+constexpr int kMaxBlocks = 32;
+
+inline unsigned nblk(int64_t n) {
+  return static_cast<unsigned>(std::min<int64_t>(std::max<int64_t>(cdiv(n, kB * 4), 1), 2048));
+}
+
+__device__ inline double wsum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// partial[b][a*K + c] = sum over this workgroup's rows of A[n][a] * B[n][c]
+template <typename TA, typename TB, int K>
+__global__ __launch_bounds__(kB) void k_gram(const TA* __restrict__ A, const TB* __restrict__ B, int64_t n,
+                                             double* __restrict__ partial) {
+  __shared__ double sh[kB / 64][K * K];
+  double acc[K][K];
+#pragma unroll
+  for (int a = 0; a < K; ++a)
+#pragma unroll
+    for (int c = 0; c < K; ++c) acc[a][c] = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    double av[K], bv[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) av[a] = static_cast<double>(A[i * K + a]);
+#pragma unroll
+    for (int c = 0; c < K; ++c) bv[c] = static_cast<double>(B[i * K + c]);
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[a][c] = __fma_rn(av[a], bv[c], acc[a][c]);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < K; ++a)
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const double s = wsum(acc[a][c]);
+      if (lane == 0) sh[w][a * K + c] = s;
+    }
+  __syncthreads();
+  for (int q = threadIdx.x; q < K * K; q += kB) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < kB / 64; ++ww) s += sh[ww][q];
+    partial[static_cast<int64_t>(blockIdx.x) * K * K + q] = s;
+  }
+}
+
+// out[q] = sum_b partial[b][q], one workgroup per q
+__global__ __launch_bounds__(kB) void k_sum_partials(const double* __restrict__ partial, int nblock, int m,
+                                                    double* __restrict__ out) {
+  __shared__ double sh[kB / 64];
+  const int q = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nblock; b += kB) s += partial[static_cast<int64_t>(b) * m + q];
+  s = wsum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int w = 0; w < kB / 64; ++w) r += sh[w];
+    out[q] = r;
+  }
+}
+
+struct BlockPtrs {
+  const void* b[kMaxBlocks];
+  const void* hb[kMaxBlocks];
+};
+
+// Subspace combination per determinant: y = X u_0 + sum_{i>=1} B_i u_i, hy likewise from the
+// sigma blocks (u_i: K x K blocks of coef, row-major [a][v]).  mode 0 writes the residual
+// hy - y diag(e), mode 1 the Ritz vectors y.  Workgroup partial sums of |out_v|^2.
+template <typename P, int K>
+__global__ __launch_bounds__(kB) void k_combine(const double* __restrict__ X, const double* __restrict__ HX,
+                                                BlockPtrs bp, int nblocks, const double* __restrict__ coef,
+                                                const double* __restrict__ e, int mode, double* __restrict__ out,
+                                                int64_t n, double* __restrict__ partial) {
+  __shared__ double cs[kMaxBlocks * K * K];
+  __shared__ double es[K];
+  __shared__ double sh[kB / 64][K];
+  for (int q = threadIdx.x; q < nblocks * K * K; q += kB) cs[q] = coef[q];
+  if (threadIdx.x < K) es[threadIdx.x] = e[threadIdx.x];
+  __syncthreads();
+  double nrm[K];
+#pragma unroll
+  for (int v = 0; v < K; ++v) nrm[v] = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    double y[K], hy[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) y[v] = hy[v] = 0.0;
+    {
+      double xa[K], ha[K];
+#pragma unroll
+      for (int a = 0; a < K; ++a) {
+        xa[a] = X[i * K + a];
+        ha[a] = mode == 0 ? HX[i * K + a] : 0.0;
+      }
+#pragma unroll
+      for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int v = 0; v < K; ++v) {
+          y[v] = __fma_rn(xa[a], cs[a * K + v], y[v]);
+          hy[v] = __fma_rn(ha[a], cs[a * K + v], hy[v]);
+        }
+    }
+    for (int blk = 1; blk < nblocks; ++blk) {
+      const P* B = static_cast<const P*>(bp.b[blk]);
+      const P* HB = static_cast<const P*>(bp.hb[blk]);
+      const double* cb = cs + blk * K * K;
+      double xa[K], ha[K];
+#pragma unroll
+      for (int a = 0; a < K; ++a) {
+        xa[a] = static_cast<double>(B[i * K + a]);
+        ha[a] = mode == 0 ? static_cast<double>(HB[i * K + a]) : 0.0;
+      }
+#pragma unroll
+      for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int v = 0; v < K; ++v) {
+          y[v] = __fma_rn(xa[a], cb[a * K + v], y[v]);
+          hy[v] = __fma_rn(ha[a], cb[a * K + v], hy[v]);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < K; ++v) {
+      const double o = mode == 0 ? hy[v] - es[v] * y[v] : y[v];
+      out[i * K + v] = o;
+      nrm[v] = __fma_rn(o, o, nrm[v]);
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int v = 0; v < K; ++v) {
+    const double a = wsum(nrm[v]);
+    if (lane == 0) sh[w][v] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double s = 0.0;
+    for (int ww = 0; ww < kB / 64; ++ww) s += sh[ww][threadIdx.x];
+    partial[static_cast<int64_t>(blockIdx.x) * K + threadIdx.x] = s;
+  }
+}
+
+// T = R / (e_v - H_ii) with the reference's regulariser (davidson.h:855-862), stored in P
+template <typename P, int K>
+__global__ __launch_bounds__(kB) void k_precond_diag(const double* __restrict__ R, const double* __restrict__ e,
+                                                     const double* __restrict__ hii, P* __restrict__ T, int64_t n) {
+  double es[K];
+#pragma unroll
+  for (int v = 0; v < K; ++v) es[v] = e[v];
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    const double h = hii[i];
+#pragma unroll
+    for (int v = 0; v < K; ++v) {
+      const double den = es[v] - h;
+      const double d = fabs(den) > 1.0e-12 ? den : den - 1.0e-12;
+      T[i * K + v] = static_cast<P>(R[i * K + v] / d);
+    }
+  }
+}
+
+// T -= X c_0 + sum_{i>=1} B_i c_i (the projection step of classical Gram-Schmidt), FP64 math
+template <typename P, int K>
+__global__ __launch_bounds__(kB) void k_project_out(P* __restrict__ T, const double* __restrict__ X, BlockPtrs bp,
+                                                    int nblocks, const double* __restrict__ coef, int64_t n) {
+  __shared__ double cs[kMaxBlocks * K * K];
+  for (int q = threadIdx.x; q < nblocks * K * K; q += kB) cs[q] = coef[q];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    double t[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) t[v] = static_cast<double>(T[i * K + v]);
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      const double xa = X[i * K + a];
+#pragma unroll
+      for (int v = 0; v < K; ++v) t[v] = __fma_rn(-xa, cs[a * K + v], t[v]);
+    }
+    for (int blk = 1; blk < nblocks; ++blk) {
+      const P* B = static_cast<const P*>(bp.b[blk]);
+      const double* cb = cs + blk * K * K;
+#pragma unroll
+      for (int a = 0; a < K; ++a) {
+        const double ba = static_cast<double>(B[i * K + a]);
+#pragma unroll
+        for (int v = 0; v < K; ++v) t[v] = __fma_rn(-ba, cb[a * K + v], t[v]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < K; ++v) T[i * K + v] = static_cast<P>(t[v]);
+  }
+}
+
+// Y <- Y M (K x K, row-major [a][v]) in place
+template <typename TY, int K>
+__global__ __launch_bounds__(kB) void k_rotate(TY* __restrict__ Y, const double* __restrict__ M, int64_t n) {
+  __shared__ double ms[K * K];
+  for (int q = threadIdx.x; q < K * K; q += kB) ms[q] = M[q];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    double y[K], o[K];
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      y[a] = static_cast<double>(Y[i * K + a]);
+      o[a] = 0.0;
+    }
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int v = 0; v < K; ++v) o[v] = __fma_rn(y[a], ms[a * K + v], o[v]);
+#pragma unroll
+    for (int v = 0; v < K; ++v) Y[i * K + v] = static_cast<TY>(o[v]);
+  }
+}
+
+// dst[i * ds + od] = src[i * ss + os] for i < n (one column of an interleaved block)
+__global__ __launch_bounds__(kB) void k_copy_col(double* __restrict__ dst, int ds, int od,
+                                                 const double* __restrict__ src, int ss, int os, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB)
+    dst[i * ds + od] = src[i * ss + os];
+}
+
+// the kq smallest H_ii over i >= 1 per workgroup (kq <= 8): values and indices, ascending
+__global__ __launch_bounds__(kB) void k_lowest(const double* __restrict__ hii, int64_t n, int kq,
+                                               double* __restrict__ val, int64_t* __restrict__ idx) {
+  __shared__ double sv[kB * 8];
+  __shared__ int64_t si[kB * 8];
+  double v[8];
+  int64_t ix[8];
+  for (int a = 0; a < 8; ++a) {
+    v[a] = 1e300;
+    ix[a] = -1;
+  }
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x + 1; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    const double h = hii[i];
+    if (h < v[kq - 1]) {
+      int p = kq - 1;
+      while (p > 0 && v[p - 1] > h) {
+        v[p] = v[p - 1];
+        ix[p] = ix[p - 1];
+        --p;
+      }
+      v[p] = h;
+      ix[p] = i;
+    }
+  }
+  for (int a = 0; a < kq; ++a) {
+    sv[threadIdx.x * 8 + a] = v[a];
+    si[threadIdx.x * 8 + a] = ix[a];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double bv[8];
+    int64_t bi[8];
+    for (int a = 0; a < 8; ++a) {
+      bv[a] = 1e300;
+      bi[a] = -1;
+    }
+    for (int t = 0; t < kB; ++t)
+      for (int a = 0; a < kq; ++a) {
+        const double h = sv[t * 8 + a];
+        const int64_t hi = si[t * 8 + a];
+        if (hi < 0 || !(h < bv[kq - 1])) continue;
+        int p = kq - 1;
+        while (p > 0 && bv[p - 1] > h) {
+          bv[p] = bv[p - 1];
+          bi[p] = bi[p - 1];
+          --p;
+        }
+        bv[p] = h;
+        bi[p] = hi;
+      }
+    for (int a = 0; a < kq; ++a) {
+      val[blockIdx.x * kq + a] = bv[a];
+      idx[blockIdx.x * kq + a] = bi[a];
+    }
+  }
+}
+
+__global__ void k_set_entries(double* __restrict__ X, const int64_t* __restrict__ pos, int m) {
+  const int q = threadIdx.x;
+  if (q < m) X[pos[q]] = 1.0;
+}
+
+double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <typename P, int K>
+struct BlockSolver {
+  sbdx_ctx& c;
+  const sbdx_davidson_opts& o;
+  hipStream_t st;
+  const int64_t n;
+  const int m;                      // blocks in a full subspace: anchor + m-1 corrections
+  const int nroots;                 // roots that must converge (<= K)
+  DevBuf<double> X, HX, R;          // anchor, its FP64 sigma, residual / next anchor
+  std::vector<DevBuf<P>> B, HB;     // corrections 1..m-1 and their sigmas (B[0] unused)
+  DevBuf<P> T;                      // block under construction
+  DevBuf<double> partial, small;    // reduction partials; staged coefficients
+  std::vector<double> Hs, Ss;       // (m K) x (m K) subspace matrices, row-major
+  static constexpr int prec = std::is_same<P, float>::value ? SBDX_FP32 : SBDX_FP64;
+
+  BlockSolver(sbdx_ctx& cc, const sbdx_davidson_opts& oo)
+      : c(cc), o(oo), st(cc.stream), n(cc.ndet()),
+        m(std::max(2, std::min(oo.num_block, kMaxBlocks))), nroots(std::max(1, oo.nroots)) {
+    X.alloc(n * K);
+    HX.alloc(n * K);
+    R.alloc(n * K);
+    T.alloc(n * K);
+    B.resize(m);
+    HB.resize(m);
+    for (int i = 1; i < m; ++i) {
+      B[i].alloc(n * K);
+      HB[i].alloc(n * K);
+    }
+    partial.alloc(static_cast<size_t>(2048) * K * K);
+    small.alloc(kMaxBlocks * K * K + K);
+    Hs.assign(static_cast<size_t>(m) * K * m * K, 0.0);
+    Ss = Hs;
+  }
+
+  void upload(double* dst, const double* src, size_t count) {
+    SBDX_HIP(hipMemcpyAsync(dst, src, count * 8, hipMemcpyHostToDevice, st));
+    SBDX_HIP(hipStreamSynchronize(st));
+  }
+  const void* bptr(int i) const { return i == 0 ? static_cast<const void*>(X.get()) : B[i].get(); }
+  const void* hbptr(int i) const { return i == 0 ? static_cast<const void*>(HX.get()) : HB[i].get(); }
+  BlockPtrs ptrs(int nblocks) const {
+    BlockPtrs p{};
+    for (int i = 0; i < nblocks; ++i) {
+      p.b[i] = bptr(i);
+      p.hb[i] = hbptr(i);
+    }
+    return p;
+  }
+
+  // out (host, K x K row-major) = A^T B
+  template <typename TA, typename TB>
+  void gram(const TA* a, const TB* b, double* out) {
+    const unsigned g = nblk(n);
+    k_gram<TA, TB, K><<<g, kB, 0, st>>>(a, b, n, partial.get());
+    SBDX_CHECK_LAUNCH();
+    k_sum_partials<<<K * K, kB, 0, st>>>(partial.get(), static_cast<int>(g), K * K, small.get());
+    SBDX_CHECK_LAUNCH();
+    SBDX_HIP(hipMemcpyAsync(out, small.get(), K * K * 8, hipMemcpyDeviceToHost, st));
+    SBDX_HIP(hipStreamSynchronize(st));
+  }
+  template <typename TB>
+  void gram_with(int i, const TB* b, double* out) {  // B_i^T b
+    if (i == 0) gram(X.get(), b, out);
+    else gram(static_cast<const P*>(bptr(i)), b, out);
+  }
+  // subspace matrices: blocks (i, j) and (j, i) from B_i^T HB_j and B_i^T B_j
+  void fill_block(int i, int j) {
+    std::vector<double> h(K * K), s(K * K);
+    if (j == 0) {
+      gram_with(i, HX.get(), h.data());
+      gram_with(i, X.get(), s.data());
+    } else {
+      gram_with(i, static_cast<const P*>(hbptr(j)), h.data());
+      gram_with(i, static_cast<const P*>(bptr(j)), s.data());
+    }
+    const int ld = m * K;
+    for (int a = 0; a < K; ++a)
+      for (int b = 0; b < K; ++b) {
+        Hs[(i * K + a) * ld + j * K + b] = Hs[(j * K + b) * ld + i * K + a] = h[a * K + b];
+        Ss[(i * K + a) * ld + j * K + b] = Ss[(j * K + b) * ld + i * K + a] = s[a * K + b];
+      }
+  }
+
+  // lowest K Ritz pairs of the first nblocks blocks; U: (nblocks K) x K row-major
+  bool rayleigh_ritz(int nblocks, std::vector<double>& U, std::vector<double>& E) {
+    const int d = nblocks * K, ld = m * K;
+    std::vector<double> S(d * d), Hm(d * d), sw(d);
+    for (int a = 0; a < d; ++a)
+      for (int b = 0; b < d; ++b) {
+        S[a + d * b] = Ss[a * ld + b];
+        Hm[a * d + b] = 0.5 * (Hs[a * ld + b] + Hs[b * ld + a]);
+      }
+    jacobi_syev_host(d, S.data(), d, sw.data());  // S = Q diag(sw) Q^T, Q in the columns of S
+    const double smax = sw[d - 1];
+    std::vector<int> keep;
+    for (int a = 0; a < d; ++a)
+      if (sw[a] > 1e-12 * smax) keep.push_back(a);
+    const int r = static_cast<int>(keep.size());
+    if (r < K) return false;
+    std::vector<double> Z(d * r);  // Z = Q_keep diag(sw)^-1/2, d x r row-major
+    for (int a = 0; a < d; ++a)
+      for (int q = 0; q < r; ++q) Z[a * r + q] = S[a + d * keep[q]] / std::sqrt(sw[keep[q]]);
+    std::vector<double> HZ(d * r, 0.0), Hr(r * r, 0.0), ew(r);
+    for (int a = 0; a < d; ++a)
+      for (int q = 0; q < r; ++q) {
+        double s = 0.0;
+        for (int b = 0; b < d; ++b) s += Hm[a * d + b] * Z[b * r + q];
+        HZ[a * r + q] = s;
+      }
+    for (int p = 0; p < r; ++p)
+      for (int q = p; q < r; ++q) {
+        double s = 0.0;
+        for (int a = 0; a < d; ++a) s += Z[a * r + p] * HZ[a * r + q];
+        Hr[p + r * q] = Hr[q + r * p] = s;
+      }
+    jacobi_syev_host(r, Hr.data(), r, ew.data());
+    U.assign(d * K, 0.0);
+    E.assign(K, 0.0);
+    for (int v = 0; v < K; ++v) {
+      E[v] = ew[v];
+      for (int a = 0; a < d; ++a) {
+        double s = 0.0;
+        for (int q = 0; q < r; ++q) s += Z[a * r + q] * Hr[q + r * v];
+        U[a * K + v] = s;
+      }
+    }
+    return true;
+  }
+
+  // R <- residuals (mode 0) or Ritz vectors (mode 1) of coefficients U; returns column norms
+  void combine(int nblocks, const std::vector<double>& U, const std::vector<double>& E, int mode,
+               std::vector<double>& nrm) {
+    std::vector<double> stage(nblocks * K * K + K);
+    for (int i = 0; i < nblocks; ++i)
+      for (int a = 0; a < K; ++a)
+        for (int v = 0; v < K; ++v) stage[i * K * K + a * K + v] = U[(i * K + a) * K + v];
+    for (int v = 0; v < K; ++v) stage[nblocks * K * K + v] = E[v];
+    upload(small.get(), stage.data(), stage.size());
+    const unsigned g = nblk(n);
+    k_combine<P, K><<<g, kB, 0, st>>>(X.get(), HX.get(), ptrs(nblocks), nblocks, small.get(),
+                                      small.get() + nblocks * K * K, mode, R.get(), n, partial.get());
+    SBDX_CHECK_LAUNCH();
+    std::vector<double> h(static_cast<size_t>(g) * K);
+    SBDX_HIP(hipMemcpyAsync(h.data(), partial.get(), h.size() * 8, hipMemcpyDeviceToHost, st));
+    SBDX_HIP(hipStreamSynchronize(st));
+    nrm.assign(K, 0.0);
+    for (unsigned b = 0; b < g; ++b)
+      for (int v = 0; v < K; ++v) nrm[v] += h[b * K + v];
+    for (int v = 0; v < K; ++v) nrm[v] = std::sqrt(nrm[v]);
+  }
+
+  // Y <- Y M with M = Q diag(l)^-1/2 from Y^T Y = Q diag(l) Q^T (directions with l below
+  // 1e-14 max(l) become zero columns)
+  template <typename TY>
+  void loewdin(TY* Y) {
+    std::vector<double> G(K * K), Gc(K * K), w(K), M(K * K, 0.0);
+    gram(Y, Y, G.data());
+    for (int a = 0; a < K; ++a)
+      for (int b = 0; b < K; ++b) Gc[a + K * b] = G[a * K + b];
+    jacobi_syev_host(K, Gc.data(), K, w.data());
+    const double lmax = std::max(w[K - 1], 1e-300);
+    for (int a = 0; a < K; ++a)
+      for (int v = 0; v < K; ++v) M[a * K + v] = w[v] > 1e-14 * lmax ? Gc[a + K * v] / std::sqrt(w[v]) : 0.0;
+    upload(small.get(), M.data(), M.size());
+    k_rotate<TY, K><<<nblk(n), kB, 0, st>>>(Y, small.get(), n);
+    SBDX_CHECK_LAUNCH();
+  }
+
+  // new correction block in T from the residual block in R
+  void expand(int nblocks, const std::vector<double>& E) {
+    upload(small.get(), E.data(), K);
+    k_precond_diag<P, K><<<nblk(n), kB, 0, st>>>(R.get(), small.get(), c.hii.get(), T.get(), n);
+    SBDX_CHECK_LAUNCH();
+    for (int pass = 0; pass < 2; ++pass) {
+      std::vector<double> coef(nblocks * K * K);
+      for (int i = 0; i < nblocks; ++i) gram_with(i, static_cast<const P*>(T.get()), coef.data() + i * K * K);
+      upload(small.get(), coef.data(), coef.size());
+      k_project_out<P, K><<<nblk(n), kB, 0, st>>>(T.get(), X.get(), ptrs(nblocks), nblocks, small.get(), n);
+      SBDX_CHECK_LAUNCH();
+      if (pass == 0) loewdin(T.get());  // rescale before the second pass
+    }
+    loewdin(T.get());
+  }
+
+  void initial_block(const double* dW0) {
+    SBDX_HIP(hipMemsetAsync(X.get(), 0, n * K * 8, st));
+    if (K > 1) {
+      // columns 1..K-1: unit vectors on the K-1 lowest H_ii other than determinant 0
+      const int kq = K - 1;
+      const unsigned g = nblk(n);
+      DevBuf<double> val(g * kq);
+      DevBuf<int64_t> idx(g * kq);
+      k_lowest<<<g, kB, 0, st>>>(c.hii.get(), n, kq, val.get(), idx.get());
+      SBDX_CHECK_LAUNCH();
+      std::vector<double> hv(g * kq);
+      std::vector<int64_t> hi(g * kq);
+      SBDX_HIP(hipMemcpyAsync(hv.data(), val.get(), hv.size() * 8, hipMemcpyDeviceToHost, st));
+      SBDX_HIP(hipMemcpyAsync(hi.data(), idx.get(), hi.size() * 8, hipMemcpyDeviceToHost, st));
+      SBDX_HIP(hipStreamSynchronize(st));
+      std::vector<std::pair<double, int64_t>> cand;
+      for (size_t q = 0; q < hv.size(); ++q)
+        if (hi[q] >= 0) cand.emplace_back(hv[q], hi[q]);
+      std::sort(cand.begin(), cand.end());
+      if (static_cast<int>(cand.size()) < kq) throw Error(SBDX_ERR_ARG, "fewer determinants than the block width");
+      std::vector<int64_t> pos(kq);
+      for (int v = 1; v < K; ++v) pos[v - 1] = cand[v - 1].second * K + v;
+      DevBuf<int64_t> dp(kq);
+      SBDX_HIP(hipMemcpyAsync(dp.get(), pos.data(), kq * 8, hipMemcpyHostToDevice, st));
+      k_set_entries<<<1, 64, 0, st>>>(X.get(), dp.get(), kq);
+      SBDX_CHECK_LAUNCH();
+      SBDX_HIP(hipStreamSynchronize(st));
+    }
+    k_copy_col<<<nblk(n), kB, 0, st>>>(X.get(), K, 0, dW0, 1, 0, n);
+    SBDX_CHECK_LAUNCH();
+    loewdin(X.get());
+  }
+
+  void run(double* dW, sbdx_davidson_result& res, double* trace, double* resid, int trace_cap) {
+    const double t0 = now();
+    res.seconds_to_tol = -1.0;
+    res.steps_to_tol = -1;
+    initial_block(dW);
+    int nstep = 0;
+    bool conv = false, stop = false;
+    std::vector<double> U, E(K, 0.0), nr(K, 0.0), ny;
+    for (int it = 0; it < o.max_iteration && !stop; ++it) {
+      // anchor: exact FP64 sigma of X
+      sigma_block(c, SBDX_FP64, K, X.get(), HX.get(), st);
+      std::fill(Hs.begin(), Hs.end(), 0.0);
+      std::fill(Ss.begin(), Ss.end(), 0.0);
+      fill_block(0, 0);
+      int nblocks = 1;
+      for (;;) {
+        if (!rayleigh_ritz(nblocks, U, E)) throw Error(SBDX_ERR_DEVICE, "Davidson subspace lost its rank");
+        const bool exact = nblocks == 1;
+        combine(nblocks, U, E, 0, nr);
+        if (nstep < trace_cap) {
+          if (trace) trace[nstep] = E[0];
+          if (resid) resid[nstep] = nr[0];
+        }
+        ++nstep;
+        if (o.verbose) {
+          std::printf(" Davidson iteration %d.%d (tol=%.16g)%s:", it, nblocks - 1, nr[0], exact ? "" : " (mixed)");
+          for (int v = 0; v < std::min(K, 4); ++v) std::printf(" %.16g", E[v]);
+          std::printf("\n");
+          std::fflush(stdout);
+        }
+        double worst = 0.0;
+        for (int v = 0; v < nroots; ++v) worst = std::max(worst, nr[v]);
+        const bool near_ref = o.energy_ref != 0.0 && std::fabs(E[0] - o.energy_ref) < o.energy_tol;
+        if (exact) {
+          if (near_ref && res.seconds_to_tol < 0) {
+            res.seconds_to_tol = now() - t0;
+            res.steps_to_tol = nstep;
+            if (o.stop_at_tol) stop = true;
+          }
+          if (worst < o.eps) conv = stop = true;
+        }
+        if (stop || now() - t0 > o.max_time) {
+          stop = true;
+          break;
+        }
+        // a mixed-precision estimate that meets a criterion is confirmed by a restart
+        if (!exact && (worst < o.eps || (near_ref && res.seconds_to_tol < 0))) break;
+        if (nblocks == m) break;
+        expand(nblocks, E);
+        std::swap(B[nblocks], T);
+        sigma_block(c, prec, K, B[nblocks].get(), HB[nblocks].get(), st);
+        for (int i = 0; i <= nblocks; ++i) fill_block(i, nblocks);
+        ++nblocks;
+      }
+      // thick restart (or the final vectors): X <- normalised Ritz vectors of the subspace
+      combine(nblocks, U, E, 1, ny);
+      std::vector<double> M(K * K, 0.0);
+      for (int v = 0; v < K; ++v) M[v * K + v] = ny[v] > 0 ? 1.0 / ny[v] : 0.0;
+      upload(small.get(), M.data(), M.size());
+      k_rotate<double, K><<<nblk(n), kB, 0, st>>>(R.get(), small.get(), n);
+      SBDX_CHECK_LAUNCH();
+      std::swap(X, R);
+    }
+    // return the Ritz vectors vector-major in dW
+    for (int v = 0; v < K; ++v) {
+      k_copy_col<<<nblk(n), kB, 0, st>>>(dW + v * n, 1, 0, X.get(), K, v, n);
+      SBDX_CHECK_LAUNCH();
+    }
+    SBDX_HIP(hipStreamSynchronize(st));
+    res.seconds = now() - t0;
+    res.steps = nstep;
+    res.residual = nr[0];
+    res.converged = conv ? 1 : 0;
+    res.energy = E[0];
+  }
+};
+
+template <typename P>
+void run_width(sbdx_ctx& c, const sbdx_davidson_opts& o, int k, double* dW, sbdx_davidson_result& res,
+               double* trace, double* resid, int cap) {
+  switch (k) {
+    case 1: { BlockSolver<P, 1> s(c, o); s.run(dW, res, trace, resid, cap); break; }
+    case 2: { BlockSolver<P, 2> s(c, o); s.run(dW, res, trace, resid, cap); break; }
+    case 4: { BlockSolver<P, 4> s(c, o); s.run(dW, res, trace, resid, cap); break; }
+    case 8: { BlockSolver<P, 8> s(c, o); s.run(dW, res, trace, resid, cap); break; }
+    default: throw Error(SBDX_ERR_ARG, "block width must be 1, 2, 4 or 8");
+  }
+}
+
+}  // namespace
+
+// vectors per block: the next power of two >= max(nroots, block)
+int davidson_block_width(const sbdx_davidson_opts& o) {
+  const int k = std::max(std::max(1, o.nroots), o.block);
+  int w = 1;
+  while (w < k) w <<= 1;
+  return w;
+}
+
+// dW: block-width * ndet doubles (blocked basis, vector-major); vector 0 holds the initial guess
+// on entry, the Ritz vectors (lowest first) on return.
+void davidson_block(sbdx_ctx& c, const sbdx_davidson_opts& o, double* dW, sbdx_davidson_result& res,
+                    double* trace, double* resid, int trace_cap) {
+  const int k = davidson_block_width(o);
+  if (o.precision == SBDX_FP32) {
+    ensure_fp32(c);
+    run_width<float>(c, o, k, dW, res, trace, resid, trace_cap);
+  } else {
+    run_width<double>(c, o, k, dW, res, trace, resid, trace_cap);
+  }
+}
+
+}  // namespace sbdx

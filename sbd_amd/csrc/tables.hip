@@ -275,7 +275,7 @@ void build_integral_tables(sbdx_ctx& c) {
       c.hii.get(), nullptr);
   SBDX_CHECK_LAUNCH();
   SBDX_HIP(hipStreamSynchronize(st));
-  build_mix_tiles(c);
+  reset_mix_tiles(c);
   c.have_ints = true;
 }
 

@@ -73,6 +73,25 @@ def test_sigma_fp32_device(pair):
     assert rel <= 2e-6, rel
 
 
+@pytest.mark.parametrize("k", [2, 4, 8])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_sigma_block_interleaved(pair, k, prec):
+    """The fused k-vector sigma equals k single sigmas (interleaved layout [det * k + v])."""
+    torch = pytest.importorskip("torch")
+    _, inst, so, eng = pair
+    X = np.random.default_rng(10 + k).standard_normal((so.ndet, k))
+    dt = torch.float64 if prec == 0 else torch.float32
+    dx = torch.tensor(X, dtype=dt, device="cuda").contiguous()
+    dy = torch.empty_like(dx)
+    eng.sigma_block_device(prec, k, dx.data_ptr(), dy.data_ptr())
+    Y = dy.cpu().numpy().astype(np.float64)
+    tol = 1e-12 if prec == 0 else 2e-6
+    for v in range(k):
+        ref = so.mult(X[:, v])
+        rel = np.max(np.abs(Y[:, v] - ref)) / np.max(np.abs(ref))
+        assert rel <= tol, (v, rel)
+
+
 def test_davidson_energy(pair):
     name, inst, so, eng = pair
     if so.ndet > 20000:
