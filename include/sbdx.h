@@ -119,6 +119,8 @@ typedef struct {
   double seconds;         /* wall time of the iteration */
   double seconds_to_tol;  /* time at which |E0 - energy_ref| < energy_tol, or -1 */
   int steps_to_tol;
+  int precond_fallback;   /* preconditioner 1 stalled and the diagonal one took over at this
+                             step (0: never) */
 } sbdx_davidson_result;
 
 void sbdx_davidson_defaults(sbdx_davidson_opts *opts);
@@ -141,6 +143,66 @@ int sbdx_rdm(sbdx_ctx *ctx, const double *W, double *onebody, double *twobody);
 
 /* Occupations density[2p + s] = <n_{p,s}> of W (host). */
 int sbdx_occupation(sbdx_ctx *ctx, const double *W, double *density);
+
+/* ---- Learned preconditioner (no counterpart in the reference; its fallback, the diagonal
+ * preconditioner, is the reference's, chemistry/tpb/davidson.h:855-862).
+ *
+ * A 3-layer MLP of width 64 maps the features of determinant i -- occupation bits of its alpha
+ * and beta strings, xh = asinh(H_ii - theta), xr = asinh(|r_i| / rms(r)) -- to m_i, and the
+ * correction becomes t_i = r_i / (theta - H_ii) * (1 + m_i).  Parameters: one float vector of
+ * SBDX_MLP_NPARAM entries, used rounded to bfloat16 (weights BF16, accumulation FP32):
+ *   W1[130][64] (rows: alpha orbital p, 64 + beta orbital p, 128 = xh, 129 = xr), b1[64],
+ *   W2[64][64] ([in][out]), b2[64], w3[64], b3.
+ *   z1 = b1 + x W1; a1 = relu(z1); z2 = b2 + a1 W2; a2 = relu(z2); m = b3 + a2 . w3.
+ * Training pairs come from the solver's own iterates: for an iterate x and the converged
+ * vector x*, d = x* - (x . x*) x is the exact correction, t = r / (theta - H_ii) the diagonal
+ * one; u = t / |c d|, v = c d / |c d| with c = (t . d) / (d . d); loss = mean (u (1 + m) - v)^2.
+ */
+#define SBDX_MLP_NPARAM 12609
+
+typedef struct {
+  uint64_t alpha, beta;   /* occupation bit masks of the determinant's strings */
+  float xh, xr;           /* features (see above) */
+  float u, v;             /* diagonal correction and target, importance-weighted */
+} sbdx_mlp_sample;
+
+typedef struct {
+  int epochs;
+  int batch;              /* samples per optimizer step */
+  double lr, beta1, beta2, eps, weight_decay;   /* Adam */
+  uint64_t seed;          /* shuffling */
+  int verbose;
+} sbdx_mlp_train_opts;
+
+void sbdx_mlp_train_defaults(sbdx_mlp_train_opts *opts);
+
+/* Record training pairs at iterate dX against the converged dXstar (device vectors of ndet
+ * doubles, the caller's basis; need not be normalised).  Determinant i is kept with probability
+ * p_i = min(1, target * (u_i^2 + v_i^2) / sum_j (u_j^2 + v_j^2)) (a counter-based hash of i and
+ * seed decides), and its u, v are scaled by 1 / sqrt(p_i).  Writes min(count, cap) samples to
+ * out (host) in no particular order; *count = number kept; *theta = Rayleigh quotient of dX. */
+int sbdx_mlp_record(sbdx_ctx *ctx, const void *dX, const void *dXstar, double target,
+                    uint64_t seed, sbdx_mlp_sample *out, int64_t cap, int64_t *count,
+                    double *theta);
+
+/* Mean loss over n samples (host) and its gradient with respect to the bfloat16-rounded
+ * parameters (current HIP device). */
+int sbdx_mlp_loss_grad(const sbdx_mlp_sample *samples, int64_t n, const float *params,
+                       double *loss, float *grad);
+
+/* Network output m for n samples. */
+int sbdx_mlp_forward(const sbdx_mlp_sample *samples, int64_t n, const float *params, float *m);
+
+/* Adam training on the GPU; params: in = initial, out = trained (FP32 master values).
+ * loss_trace (optional): mean loss of each epoch. */
+int sbdx_mlp_train(const sbdx_mlp_sample *samples, int64_t n, const sbdx_mlp_train_opts *opts,
+                   float *params, double *loss_trace, int trace_cap);
+
+/* Network used by preconditioner 1 on this context (NULL removes it). */
+int sbdx_mlp_set(sbdx_ctx *ctx, const float *params);
+
+/* Apply preconditioner 1 to one residual (host vectors of ndet doubles, caller's basis). */
+int sbdx_mlp_apply(sbdx_ctx *ctx, double theta, const double *r, double *t);
 
 /* Measurement hook, no counterpart in the reference: when enabled, every sigma build records
  * HIP events on its launch stream around its phases (0 alpha rows, 1 alpha-single x

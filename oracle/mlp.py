@@ -10,7 +10,7 @@ Network (parameters: one float32 vector of NPARAM entries, used rounded to bfloa
     z1 = b1 + sum_{p in alpha} W1[p] + sum_{p in beta} W1[64 + p] + xh W1[128] + xr W1[129]
     a1 = relu(z1);  z2 = b2 + a1 W2;  a2 = relu(z2);  m = b3 + a2 . w3
 Preconditioned correction of residual component r_i at Ritz value theta:
-    t_i = r_i / (theta - H_ii) * (1 + m_i),  xh = asinh(H_ii - theta),  xr = asinh(r_i / rms(r))
+    t_i = r_i / (theta - H_ii) * (1 + m_i),  xh = asinh(H_ii - theta),  xr = asinh(|r_i| / rms(r))
 Training pair of determinant i (u, v already carry the importance weight and the snapshot
 normalisation): loss = (1 / n) sum_s (u_s (1 + m_s) - v_s)^2.
 """
@@ -28,7 +28,7 @@ OFF_W3 = OFF_B2 + H
 OFF_B3 = OFF_W3 + H
 NPARAM = OFF_B3 + 1
 
-SAMPLE_DTYPE = np.dtype([("a", "<u8"), ("b", "<u8"), ("xh", "<f4"), ("xr", "<f4"),
+SAMPLE_DTYPE = np.dtype([("alpha", "<u8"), ("beta", "<u8"), ("xh", "<f4"), ("xr", "<f4"),
                          ("u", "<f4"), ("v", "<f4")])
 
 
@@ -49,14 +49,15 @@ def unpack(params):
 
 
 def inputs(samples):
-    """Dense input matrix (n, 130): alpha bits, beta bits, xh, xr."""
+    """Dense input matrix (n, 130): alpha bits, beta bits, xh, xr (features rounded to
+    bfloat16, like the parameters)."""
     n = samples.size
     X = np.zeros((n, NIN))
     for p in range(64):
-        X[:, p] = (samples["a"] >> np.uint64(p)) & np.uint64(1)
-        X[:, 64 + p] = (samples["b"] >> np.uint64(p)) & np.uint64(1)
-    X[:, 128] = samples["xh"]
-    X[:, 129] = samples["xr"]
+        X[:, p] = (samples["alpha"] >> np.uint64(p)) & np.uint64(1)
+        X[:, 64 + p] = (samples["beta"] >> np.uint64(p)) & np.uint64(1)
+    X[:, 128] = bf16_round(samples["xh"])
+    X[:, 129] = bf16_round(samples["xr"])
     return X
 
 
@@ -117,20 +118,75 @@ def init_params(seed=0, zero_last=True):
 
 
 def precondition(params, alpha, beta, hii, theta, r):
-    """t = r / (theta - H_ii) * (1 + m) over a whole vector in the engine's blocked order
-    (det = ia * n_beta + ib); the regulariser of the diagonal preconditioner is the
-    reference's (chemistry/tpb/davidson.h:855-862)."""
+    """t = r / (theta - H_ii) * (1 + m) over a whole vector, det = ia * n_beta + ib (in either
+    sign convention of the determinant basis: the features use |r_i|, so t flips with r); the
+    regulariser of the diagonal preconditioner is the reference's (chemistry/tpb/davidson.h:
+    855-862)."""
     n = hii.size
     nb = beta.size
     ia = np.arange(n) // nb
     ib = np.arange(n) % nb
     rms = np.sqrt(np.dot(r, r) / n)
     s = np.zeros(n, dtype=SAMPLE_DTYPE)
-    s["a"] = alpha[ia]
-    s["b"] = beta[ib]
+    s["alpha"] = alpha[ia]
+    s["beta"] = beta[ib]
     s["xh"] = np.arcsinh(hii - theta).astype(np.float32)
-    s["xr"] = np.arcsinh(r / rms if rms > 0 else 0.0 * r).astype(np.float32)
+    s["xr"] = np.arcsinh(np.abs(r) / rms if rms > 0 else 0.0 * r).astype(np.float32)
     m, _ = forward(params, s)
     den = theta - hii
     den = np.where(np.abs(den) > 1e-12, den, den - 1e-12)
     return r / den * (1.0 + m)
+
+
+def hash01(i, seed):
+    """Counter-based uniform [0, 1) of determinant index i: the splitmix64 finaliser (Steele,
+    Lea, Flood 2014) of seed ^ ((i + 1) * 0x9E3779B97F4A7C15), top 53 bits."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) ^ ((np.asarray(i, np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def record_terms(sigma, hii, x, xs):
+    """Per-determinant quantities of a training snapshot (caller's basis, float64): Rayleigh
+    quotient theta, residual r of the normalised iterate, exact correction d = x*^ - (x^.x*^) x^
+    (x* sign-aligned with x), diagonal correction t, and u, v, q = u^2 + v^2 as recorded."""
+    xx = x @ x
+    hx = sigma(x)
+    theta = (x @ hx) / xx
+    sx = 1.0 / np.sqrt(xx)
+    xxs = x @ xs
+    sxs = (-1.0 if xxs < 0 else 1.0) / np.sqrt(xs @ xs)
+    proj = abs(xxs) / np.sqrt(xx * (xs @ xs))
+    xn = x * sx
+    r = hx * sx - theta * xn
+    d = xs * sxs - proj * xn
+    den = theta - hii
+    den = np.where(np.abs(den) > 1e-12, den, den - 1e-12)
+    t = r / den
+    c = (t @ d) / (d @ d)
+    vnorm = abs(c) * np.sqrt(d @ d)
+    u = t / vnorm
+    v = c * d / vnorm
+    return dict(theta=theta, r=r, d=d, t=t, u=u, v=v, q=u * u + v * v,
+                rms=np.sqrt((r @ r) / x.size))
+
+
+def record(sigma, hii, alpha, beta, x, xs, target, seed):
+    """The samples kept by the engine's recorder, sorted by determinant index."""
+    T = record_terms(sigma, hii, x, xs)
+    n = x.size
+    p = np.minimum(1.0, target * T["q"] / T["q"].sum())
+    keep = np.nonzero(hash01(np.arange(n), seed) < p)[0]
+    nb = beta.size
+    s = np.zeros(keep.size, dtype=SAMPLE_DTYPE)
+    s["alpha"] = alpha[keep // nb]
+    s["beta"] = beta[keep % nb]
+    s["xh"] = np.arcsinh(hii[keep] - T["theta"]).astype(np.float32)
+    s["xr"] = np.arcsinh(np.abs(T["r"][keep]) / T["rms"]).astype(np.float32)
+    w = 1.0 / np.sqrt(p[keep])
+    s["u"] = (T["u"][keep] * w).astype(np.float32)
+    s["v"] = (T["v"][keep] * w).astype(np.float32)
+    return s, keep, T["theta"], p

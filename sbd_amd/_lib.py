@@ -21,7 +21,12 @@ EXPORTS = [
     "sbdx_davidson_defaults",
     "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
     "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
+    "sbdx_mlp_train_defaults", "sbdx_mlp_record", "sbdx_mlp_loss_grad", "sbdx_mlp_forward",
+    "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply",
 ]
+MLP_NPARAM = 12609
+MLP_SAMPLE = np.dtype([("alpha", "<u8"), ("beta", "<u8"), ("xh", "<f4"), ("xr", "<f4"),
+                       ("u", "<f4"), ("v", "<f4")])
 PHASES = ["alpha_rows", "mixed", "transpose", "beta_rows", "add_transposed"]
 
 
@@ -36,7 +41,13 @@ class DavidsonOpts(C.Structure):
 class DavidsonResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("residual", C.c_double), ("steps", C.c_int),
                 ("converged", C.c_int), ("seconds", C.c_double), ("seconds_to_tol", C.c_double),
-                ("steps_to_tol", C.c_int)]
+                ("steps_to_tol", C.c_int), ("precond_fallback", C.c_int)]
+
+
+class MlpTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_double), ("beta1", C.c_double),
+                ("beta2", C.c_double), ("eps", C.c_double), ("weight_decay", C.c_double),
+                ("seed", C.c_uint64), ("verbose", C.c_int)]
 
 
 class SbdxError(RuntimeError):
@@ -83,8 +94,20 @@ def lib():
     L.sbdx_destroy.argtypes = [vp]
     L.sbdx_destroy.restype = None
     L.sbdx_last_error.restype = C.c_char_p
+    f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    smp = np.ctypeslib.ndpointer(dtype=MLP_SAMPLE, flags="C_CONTIGUOUS")
+    L.sbdx_mlp_train_defaults.argtypes = [C.POINTER(MlpTrainOpts)]
+    L.sbdx_mlp_train_defaults.restype = None
+    L.sbdx_mlp_record.argtypes = [vp, vp, vp, C.c_double, C.c_uint64, smp, C.c_int64,
+                                  C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+    L.sbdx_mlp_loss_grad.argtypes = [smp, C.c_int64, f32p, C.POINTER(C.c_double), f32p]
+    L.sbdx_mlp_forward.argtypes = [smp, C.c_int64, f32p, f32p]
+    L.sbdx_mlp_train.argtypes = [smp, C.c_int64, C.POINTER(MlpTrainOpts), f32p, vp, C.c_int]
+    L.sbdx_mlp_set.argtypes = [vp, vp]
+    L.sbdx_mlp_apply.argtypes = [vp, C.c_double, f64p, f64p]
     for name in EXPORTS:
-        if name not in ("sbdx_ndet", "sbdx_davidson_defaults", "sbdx_destroy", "sbdx_last_error"):
+        if name not in ("sbdx_ndet", "sbdx_davidson_defaults", "sbdx_destroy", "sbdx_last_error",
+                        "sbdx_mlp_train_defaults"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -105,6 +128,43 @@ def davidson_defaults(**kw) -> DavidsonOpts:
             raise KeyError(k)
         setattr(o, k, v)
     return o
+
+
+def mlp_loss_grad(samples, params):
+    """Mean loss and gradient of the learned preconditioner's network over samples (GPU)."""
+    s = np.ascontiguousarray(samples, dtype=MLP_SAMPLE)
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    loss = C.c_double()
+    g = np.zeros(MLP_NPARAM, np.float32)
+    check(lib().sbdx_mlp_loss_grad(s, s.size, p, C.byref(loss), g))
+    return loss.value, g
+
+
+def mlp_forward(samples, params):
+    s = np.ascontiguousarray(samples, dtype=MLP_SAMPLE)
+    m = np.zeros(s.size, np.float32)
+    check(lib().sbdx_mlp_forward(s, s.size, np.ascontiguousarray(params, dtype=np.float32), m))
+    return m
+
+
+def mlp_train_defaults(**kw) -> MlpTrainOpts:
+    o = MlpTrainOpts()
+    lib().sbdx_mlp_train_defaults(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+def mlp_train(samples, params, **opts):
+    """Adam training on the GPU; returns (trained FP32 parameters, per-epoch mean loss)."""
+    o = mlp_train_defaults(**opts)
+    s = np.ascontiguousarray(samples, dtype=MLP_SAMPLE)
+    p = np.ascontiguousarray(params, dtype=np.float32).copy()
+    trace = np.zeros(max(1, o.epochs))
+    check(lib().sbdx_mlp_train(s, s.size, C.byref(o), p, trace.ctypes.data, trace.size))
+    return p, trace[:o.epochs].copy()
 
 
 class Engine:
@@ -200,6 +260,7 @@ class Engine:
         return dict(W=W, energy=res.energy, residual=res.residual, steps=res.steps,
                     converged=bool(res.converged), seconds=res.seconds,
                     seconds_to_tol=res.seconds_to_tol, steps_to_tol=res.steps_to_tol,
+                    precond_fallback=res.precond_fallback,
                     trace=trace[:n].copy(), resid=resid[:n].copy())
 
     def davidson_device(self, dW, trace_cap=4096, **opts):
@@ -213,7 +274,32 @@ class Engine:
         return dict(energy=res.energy, residual=res.residual, steps=res.steps,
                     converged=bool(res.converged), seconds=res.seconds,
                     seconds_to_tol=res.seconds_to_tol, steps_to_tol=res.steps_to_tol,
+                    precond_fallback=res.precond_fallback,
                     trace=trace[:n].copy(), resid=resid[:n].copy())
+
+    def mlp_set(self, params):
+        """Network of preconditioner 1 (None removes it)."""
+        if params is None:
+            check(lib().sbdx_mlp_set(self._h, None))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        assert p.size == MLP_NPARAM
+        check(lib().sbdx_mlp_set(self._h, p.ctypes.data))
+
+    def mlp_apply(self, theta, r):
+        t = np.zeros(self.ndet)
+        check(lib().sbdx_mlp_apply(self._h, float(theta), np.ascontiguousarray(r, np.float64), t))
+        return t
+
+    def mlp_record(self, dX, dXstar, target, seed=0, cap=None):
+        """Training pairs at iterate dX against the converged dXstar (device pointers)."""
+        cap = int(cap if cap is not None else max(16, 4 * target))
+        out = np.zeros(cap, dtype=MLP_SAMPLE)
+        cnt = C.c_int64()
+        theta = C.c_double()
+        check(lib().sbdx_mlp_record(self._h, C.c_void_p(dX), C.c_void_p(dXstar), float(target),
+                                    int(seed), out, cap, C.byref(cnt), C.byref(theta)))
+        return out[:min(cnt.value, cap)].copy(), cnt.value, theta.value
 
     def rdm(self, W):
         L = self.norb

@@ -6,6 +6,7 @@
 #include "common.h"
 #include "sbdx.h"
 
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -18,6 +19,13 @@ void davidson_block(sbdx_ctx& c, const sbdx_davidson_opts& o, double* dW, sbdx_d
 int davidson_block_width(const sbdx_davidson_opts& o);
 void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody);
 void occupation_blocked(sbdx_ctx& c, const double* dW, double* density);
+void mlp_set(sbdx_ctx& c, const float* params);
+int64_t mlp_record(sbdx_ctx& c, const double* dX, const double* dXs, double target, uint64_t seed,
+                   sbdx_mlp_sample* out, int64_t cap, double* theta_out);
+void mlp_loss_grad(const sbdx_mlp_sample* S, int64_t n, const float* params, double* loss, float* grad);
+void mlp_forward(const sbdx_mlp_sample* S, int64_t n, const float* params, float* m);
+void mlp_train(const sbdx_mlp_sample* S, int64_t n, const sbdx_mlp_train_opts& o, float* params, double* trace,
+               int trace_cap);
 }  // namespace sbdx
 
 namespace {
@@ -233,7 +241,8 @@ static int check_opts(const sbdx_ctx* c, const sbdx_davidson_opts* o) {
   need(o->max_iteration >= 1 && o->num_block >= 1 && o->num_block <= 64, "bad iteration/block");
   need(o->nroots >= 1 && o->nroots <= 8, "nroots must be 1..8");
   need(o->precision == SBDX_FP64 || o->precision == SBDX_FP32, "precision must be SBDX_FP64 or SBDX_FP32");
-  need(o->preconditioner == 0, "preconditioner must be 0 (diagonal)");
+  need(o->preconditioner == 0 || o->preconditioner == 1, "preconditioner must be 0 (diagonal) or 1 (learned)");
+  need(o->preconditioner == 0 || c->have_mlp, "preconditioner 1 needs a network (sbdx_mlp_set)");
   need(o->method == 0 || o->method == 1, "method must be 0 or 1");
   need(o->block >= 0 && o->block <= 8, "block must be 0..8");
   if (o->method == 0) {
@@ -302,6 +311,79 @@ int sbdx_occupation(sbdx_ctx* c, const double* W, double* density) {
     sbdx::DevBuf<double> w(c->ndet()), tmp;
     upload_blocked(*c, W, w.get(), tmp);
     sbdx::occupation_blocked(*c, w.get(), density);
+  });
+}
+
+void sbdx_mlp_train_defaults(sbdx_mlp_train_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->epochs = 20;
+  o->batch = 16384;
+  o->lr = 1.0e-3;
+  o->beta1 = 0.9;
+  o->beta2 = 0.999;
+  o->eps = 1.0e-8;
+  o->weight_decay = 0.0;
+  o->seed = 0;
+  o->verbose = 0;
+}
+
+int sbdx_mlp_record(sbdx_ctx* c, const void* dX, const void* dXstar, double target, uint64_t seed,
+                    sbdx_mlp_sample* out, int64_t cap, int64_t* count, double* theta) {
+  return guarded([&] {
+    need_ints(c);
+    need(dX && dXstar && count && cap >= 0 && (out || cap == 0), "null argument");
+    need(target > 0.0, "target must be positive");
+    const int64_t n = c->ndet();
+    sbdx::DevBuf<double> x(n), xs(n);
+    sbdx::apply_sign(*c, SBDX_FP64, 1, dX, x.get(), c->stream);
+    sbdx::apply_sign(*c, SBDX_FP64, 1, dXstar, xs.get(), c->stream);
+    *count = sbdx::mlp_record(*c, x.get(), xs.get(), target, seed, out, cap, theta);
+  });
+}
+
+int sbdx_mlp_loss_grad(const sbdx_mlp_sample* samples, int64_t n, const float* params, double* loss, float* grad) {
+  return guarded([&] {
+    need(samples && params && loss && grad && n > 0, "null argument or empty sample set");
+    sbdx::mlp_loss_grad(samples, n, params, loss, grad);
+  });
+}
+
+int sbdx_mlp_forward(const sbdx_mlp_sample* samples, int64_t n, const float* params, float* m) {
+  return guarded([&] {
+    need(samples && params && m && n > 0, "null argument or empty sample set");
+    sbdx::mlp_forward(samples, n, params, m);
+  });
+}
+
+int sbdx_mlp_train(const sbdx_mlp_sample* samples, int64_t n, const sbdx_mlp_train_opts* o, float* params,
+                   double* loss_trace, int trace_cap) {
+  return guarded([&] {
+    need(samples && o && params && n > 0, "null argument or empty sample set");
+    need(o->epochs >= 0 && o->batch >= 1 && o->lr > 0.0, "bad training options");
+    sbdx::mlp_train(samples, n, *o, params, loss_trace, trace_cap);
+  });
+}
+
+int sbdx_mlp_set(sbdx_ctx* c, const float* params) {
+  return guarded([&] {
+    need(c != nullptr, "null context");
+    sbdx::mlp_set(*c, params);
+  });
+}
+
+int sbdx_mlp_apply(sbdx_ctx* c, double theta, const double* r, double* t) {
+  return guarded([&] {
+    need_ints(c);
+    need(r && t, "null argument");
+    const int64_t n = c->ndet();
+    sbdx::DevBuf<double> rb(n), tb(n), tmp;
+    upload_blocked(*c, r, rb.get(), tmp);
+    double nr = 0.0;
+    for (int64_t i = 0; i < n; ++i) nr += r[i] * r[i];
+    const double rms = std::sqrt(nr / static_cast<double>(n));
+    sbdx::mlp_precondition(*c, SBDX_FP64, 1, rb.get(), &theta, &rms, tb.get(), c->stream);
+    download_user(*c, tb.get(), t, tmp);
   });
 }
 

@@ -130,6 +130,9 @@ struct sbdx_ctx {
   sbdx::DevBuf<float> hii32;
   sbdx::Workspace ws;
   sbdx::Prof prof;
+  bool have_mlp = false;           // learned preconditioner set (sbdx_mlp_set)
+  sbdx::DevBuf<float> mlp_w;       // its parameters, rounded to bfloat16
+  sbdx::DevBuf<float> mlp_pa, mlp_pb;  // per-string first-layer sums (alpha includes b1)
   int64_t ndet() const { return na * nb; }
 };
 
@@ -207,5 +210,8 @@ void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipS
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
 void apply_sign(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 void apply_sign_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
+// learned preconditioner (mlp.hip): T[det*k+v] = R/(theta_v - H_ii) (1 + m), P = float|double
+void mlp_precondition(sbdx_ctx& c, int prec, int k, const double* R, const double* theta,
+                      const double* rms, void* T, hipStream_t s);
 
 }  // namespace sbdx

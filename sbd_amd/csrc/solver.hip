@@ -344,11 +344,14 @@ struct BlockSolver {
   DevBuf<P> T;                      // block under construction
   DevBuf<double> partial, small;    // reduction partials; staged coefficients
   std::vector<double> Hs, Ss;       // (m K) x (m K) subspace matrices, row-major
+  bool use_mlp;                     // learned preconditioner in use
+  int fallback_step = 0;            // step at which it was dropped (0: never)
   static constexpr int prec = std::is_same<P, float>::value ? SBDX_FP32 : SBDX_FP64;
 
   BlockSolver(sbdx_ctx& cc, const sbdx_davidson_opts& oo)
       : c(cc), o(oo), st(cc.stream), n(cc.ndet()),
-        m(std::max(2, std::min(oo.num_block, kMaxBlocks))), nroots(std::max(1, oo.nroots)) {
+        m(std::max(2, std::min(oo.num_block, kMaxBlocks))), nroots(std::max(1, oo.nroots)),
+        use_mlp(oo.preconditioner == 1) {
     X.alloc(n * K);
     HX.alloc(n * K);
     R.alloc(n * K);
@@ -396,12 +399,19 @@ struct BlockSolver {
     if (i == 0) gram(X.get(), b, out);
     else gram(static_cast<const P*>(bptr(i)), b, out);
   }
-  // subspace matrices: blocks (i, j) and (j, i) from B_i^T HB_j and B_i^T B_j
+  // subspace matrices: blocks (i, j) and (j, i) from B_i^T HB_j and B_i^T B_j.  The anchor's
+  // coupling to a correction block is taken as (H X)^T B_j with the exact FP64 sigma of X, not
+  // as X^T (H B_j) with the correction's precision-P sigma: it is the first-order term that
+  // decides the correction's coefficient, and its P rounding (|H| eps_P) would otherwise set
+  // a floor under the residual once |r| falls to that size.
   void fill_block(int i, int j) {
     std::vector<double> h(K * K), s(K * K);
     if (j == 0) {
       gram_with(i, HX.get(), h.data());
       gram_with(i, X.get(), s.data());
+    } else if (i == 0) {
+      gram(HX.get(), static_cast<const P*>(bptr(j)), h.data());
+      gram(X.get(), static_cast<const P*>(bptr(j)), s.data());
     } else {
       gram_with(i, static_cast<const P*>(hbptr(j)), h.data());
       gram_with(i, static_cast<const P*>(bptr(j)), s.data());
@@ -499,11 +509,29 @@ struct BlockSolver {
     SBDX_CHECK_LAUNCH();
   }
 
-  // new correction block in T from the residual block in R
-  void expand(int nblocks, const std::vector<double>& E) {
+  // preconditioned residuals in T: the learned preconditioner while it is in use (falling
+  // back for good if it yields a non-finite block), else the diagonal one
+  void precondition(const std::vector<double>& E, const std::vector<double>& nr, int nstep) {
+    if (use_mlp) {
+      std::vector<double> rms(K);
+      for (int v = 0; v < K; ++v) rms[v] = nr[v] / std::sqrt(static_cast<double>(n));
+      mlp_precondition(c, prec, K, R.get(), E.data(), rms.data(), T.get(), st);
+      std::vector<double> G(K * K);
+      gram(T.get(), T.get(), G.data());
+      bool finite = true;
+      for (double g : G) finite = finite && std::isfinite(g);
+      if (finite) return;
+      use_mlp = false;
+      fallback_step = nstep;
+    }
     upload(small.get(), E.data(), K);
     k_precond_diag<P, K><<<nblk(n), kB, 0, st>>>(R.get(), small.get(), c.hii.get(), T.get(), n);
     SBDX_CHECK_LAUNCH();
+  }
+
+  // new correction block in T from the residual block in R
+  void expand(int nblocks, const std::vector<double>& E, const std::vector<double>& nr, int nstep) {
+    precondition(E, nr, nstep);
     for (int pass = 0; pass < 2; ++pass) {
       std::vector<double> coef(nblocks * K * K);
       for (int i = 0; i < nblocks; ++i) gram_with(i, static_cast<const P*>(T.get()), coef.data() + i * K * K);
@@ -553,8 +581,9 @@ struct BlockSolver {
     res.seconds_to_tol = -1.0;
     res.steps_to_tol = -1;
     initial_block(dW);
-    int nstep = 0;
+    int nstep = 0, stalls = 0;
     bool conv = false, stop = false;
+    double best_anchor = 0.0;
     std::vector<double> U, E(K, 0.0), nr(K, 0.0), ny;
     for (int it = 0; it < o.max_iteration && !stop; ++it) {
       // anchor: exact FP64 sigma of X
@@ -582,6 +611,16 @@ struct BlockSolver {
         for (int v = 0; v < nroots; ++v) worst = std::max(worst, nr[v]);
         const bool near_ref = o.energy_ref != 0.0 && std::fabs(E[0] - o.energy_ref) < o.energy_tol;
         if (exact) {
+          // stall test of the learned preconditioner on the exact residual of every restart:
+          // two restarts in a row without a new lowest residual switch to the diagonal one
+          if (use_mlp && it > 0) {
+            stalls = worst < best_anchor ? 0 : stalls + 1;
+            if (stalls >= 2) {
+              use_mlp = false;
+              fallback_step = nstep;
+            }
+          }
+          if (it == 0 || worst < best_anchor) best_anchor = worst;
           if (near_ref && res.seconds_to_tol < 0) {
             res.seconds_to_tol = now() - t0;
             res.steps_to_tol = nstep;
@@ -596,7 +635,7 @@ struct BlockSolver {
         // a mixed-precision estimate that meets a criterion is confirmed by a restart
         if (!exact && (worst < o.eps || (near_ref && res.seconds_to_tol < 0))) break;
         if (nblocks == m) break;
-        expand(nblocks, E);
+        expand(nblocks, E, nr, nstep);
         std::swap(B[nblocks], T);
         sigma_block(c, prec, K, B[nblocks].get(), HB[nblocks].get(), st);
         for (int i = 0; i <= nblocks; ++i) fill_block(i, nblocks);
@@ -622,6 +661,7 @@ struct BlockSolver {
     res.residual = nr[0];
     res.converged = conv ? 1 : 0;
     res.energy = E[0];
+    res.precond_fallback = fallback_step;
   }
 };
 
