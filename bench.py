@@ -137,6 +137,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain 1 GiB device copy (rocprof traffic calibration)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the mixed pass of each --block width before timing it")
     ap.add_argument("--block", type=int, nargs="*", default=[],
                     help="also time the fused sigma of blocks of these widths (1, 2, 4, 8)")
     args = ap.parse_args()
@@ -265,6 +267,11 @@ def main():
         torch.cuda.empty_cache()
         out["block_sigma"] = {}
         for k in args.block:
+            tuned = None
+            if args.autotune:
+                chosen, cand = eng.autotune(prec, k, reps=1)
+                tuned = {"entries_per_thread": chosen, "candidates_ms": cand}
+                log(f"[bench] autotune block {k}: {tuned}")
             xb = torch.randn(N * k, dtype=dt, device="cuda", generator=g)
             yb = torch.empty_like(xb)
             eng.sigma_block_device(prec, k, xb.data_ptr(), yb.data_ptr(), stream)
@@ -282,6 +289,7 @@ def main():
                 "ms_per_block": round(tb * 1e3, 3),
                 "vector_sigmas_per_s": round(k / tb, 4),
                 "phase_ms": {p: round(v / max(cl, 1), 3) for p, v in ph.items()},
+                "autotune": tuned,
             }
             log(f"[bench] block {k}: {tb * 1e3:.1f} ms, {k / tb:.3f} vector sigmas/s")
             del xb, yb

@@ -204,6 +204,31 @@ int sbdx_mlp_set(sbdx_ctx *ctx, const float *params);
 /* Apply preconditioner 1 to one residual (host vectors of ndet doubles, caller's basis). */
 int sbdx_mlp_apply(sbdx_ctx *ctx, double theta, const double *r, double *t);
 
+/* ---- Batches of independent problems, no counterpart in the reference (sbd solves one problem
+ * per MPI job): many small contexts solved together by a lockstep block Davidson (one vector
+ * per block, thick restart, FP64 or mixed precision, diagonal preconditioner), each lockstep
+ * step replayed as one HIP graph that holds the kernels of every unconverged problem. */
+typedef struct sbdx_batch sbdx_batch;
+
+/* ctxs: count contexts on the current device, integrals set; the batch does not own them. */
+int sbdx_batch_create(int count, sbdx_ctx *const *ctxs, sbdx_batch **out);
+
+/* Solve every problem from its HF vector (opts: nroots 1, block 0/1, preconditioner 0,
+ * num_block 2..24; method is ignored).  W (optional): count host buffers of ndet_i doubles
+ * (NULL entries skipped) for the returned vectors; res: count results. */
+int sbdx_batch_davidson(sbdx_batch *batch, const sbdx_davidson_opts *opts, double *const *W,
+                        sbdx_davidson_result *res);
+
+void sbdx_batch_destroy(sbdx_batch *batch);
+
+/* Autotuner, no counterpart in the reference: times every variant of the sigma build's
+ * alpha-single x beta-single kernel (entries per thread, which also sets its tile shape) for
+ * block width k and precision prec on this context's problem, `reps` builds each, and keeps the
+ * fastest for every later build of that (prec, k).  candidates/ms (optional, cap entries): the
+ * variants and their mean ms per build; *chosen: the variant kept; *ncand: variants tried. */
+int sbdx_autotune(sbdx_ctx *ctx, int prec, int k, int reps, int *chosen, int *candidates,
+                  double *ms, int cap, int *ncand);
+
 /* Measurement hook, no counterpart in the reference: when enabled, every sigma build records
  * HIP events on its launch stream around its phases (0 alpha rows, 1 alpha-single x
  * beta-single, 2 transpose, 3 beta rows, 4 transposed add) and accumulates their durations.

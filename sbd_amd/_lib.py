@@ -22,7 +22,8 @@ EXPORTS = [
     "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
     "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
     "sbdx_mlp_train_defaults", "sbdx_mlp_record", "sbdx_mlp_loss_grad", "sbdx_mlp_forward",
-    "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply",
+    "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune",
+    "sbdx_batch_create", "sbdx_batch_davidson", "sbdx_batch_destroy",
 ]
 MLP_NPARAM = 12609
 MLP_SAMPLE = np.dtype([("alpha", "<u8"), ("beta", "<u8"), ("xh", "<f4"), ("xr", "<f4"),
@@ -105,9 +106,17 @@ def lib():
     L.sbdx_mlp_train.argtypes = [smp, C.c_int64, C.POINTER(MlpTrainOpts), f32p, vp, C.c_int]
     L.sbdx_mlp_set.argtypes = [vp, vp]
     L.sbdx_mlp_apply.argtypes = [vp, C.c_double, f64p, f64p]
+    i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+    L.sbdx_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), i32p, f64p,
+                                C.c_int, C.POINTER(C.c_int)]
+    L.sbdx_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp)]
+    L.sbdx_batch_davidson.argtypes = [vp, C.POINTER(DavidsonOpts), C.POINTER(vp),
+                                      C.POINTER(DavidsonResult)]
+    L.sbdx_batch_destroy.argtypes = [vp]
+    L.sbdx_batch_destroy.restype = None
     for name in EXPORTS:
         if name not in ("sbdx_ndet", "sbdx_davidson_defaults", "sbdx_destroy", "sbdx_last_error",
-                        "sbdx_mlp_train_defaults"):
+                        "sbdx_mlp_train_defaults", "sbdx_batch_destroy"):
             getattr(L, name).restype = C.c_int
     _lib = L
     return L
@@ -286,6 +295,15 @@ class Engine:
         assert p.size == MLP_NPARAM
         check(lib().sbdx_mlp_set(self._h, p.ctypes.data))
 
+    def autotune(self, prec, k, reps=2):
+        """Time the mixed-pass variants for (prec, k) and keep the fastest; returns
+        (chosen entries per thread, {candidate: mean ms})."""
+        cand = np.zeros(8, np.int32)
+        ms = np.zeros(8)
+        chosen, nc = C.c_int(), C.c_int()
+        check(lib().sbdx_autotune(self._h, prec, k, reps, C.byref(chosen), cand, ms, 8, C.byref(nc)))
+        return chosen.value, {int(cand[i]): float(ms[i]) for i in range(nc.value)}
+
     def mlp_apply(self, theta, r):
         t = np.zeros(self.ndet)
         check(lib().sbdx_mlp_apply(self._h, float(theta), np.ascontiguousarray(r, np.float64), t))
@@ -326,6 +344,45 @@ class Engine:
         if getattr(self, "_h", None):
             lib().sbdx_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several Engines (same device) solved together by the lockstep graph-replayed Davidson."""
+
+    def __init__(self, engines):
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e._h.value for e in self.engines])
+        h = C.c_void_p()
+        check(lib().sbdx_batch_create(len(self.engines), arr, C.byref(h)))
+        self._h = h
+
+    def davidson(self, want_vectors=False, **opts):
+        o = davidson_defaults(**opts)
+        cnt = len(self.engines)
+        res = (DavidsonResult * cnt)()
+        W = [np.zeros(e.ndet) for e in self.engines] if want_vectors else None
+        ptrs = (C.c_void_p * cnt)(*[w.ctypes.data for w in W]) if W else None
+        check(lib().sbdx_batch_davidson(self._h, C.byref(o), ptrs, res))
+        out = []
+        for i in range(cnt):
+            r = res[i]
+            d = dict(energy=r.energy, residual=r.residual, steps=r.steps, converged=bool(r.converged),
+                     seconds=r.seconds, seconds_to_tol=r.seconds_to_tol, steps_to_tol=r.steps_to_tol)
+            if W:
+                d["W"] = W[i]
+            out.append(d)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().sbdx_batch_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __del__(self):
         try:

@@ -387,6 +387,94 @@ int sbdx_mlp_apply(sbdx_ctx* c, double theta, const double* r, double* t) {
   });
 }
 
+int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* candidates, double* ms, int cap,
+                  int* ncand) {
+  return guarded([&] {
+    need_ints(c);
+    need((k == 1 || k == 2 || k == 4 || k == 8) && (prec == SBDX_FP64 || prec == SBDX_FP32) && reps >= 1,
+         "bad argument (k must be 1, 2, 4 or 8)");
+    const int64_t n = c->ndet();
+    const size_t es = prec == SBDX_FP64 ? 8 : 4;
+    sbdx::DevBuf<unsigned char> x(n * k * es), y(n * k * es);
+    SBDX_HIP(hipMemsetAsync(x.get(), 0, n * k * es, c->stream));
+    hipEvent_t e0, e1;
+    SBDX_HIP(hipEventCreate(&e0));
+    SBDX_HIP(hipEventCreate(&e1));
+    const std::vector<int> cand = sbdx::mix_ept_candidates(k);
+    int& slot = c->mix_ept_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)];
+    int best = 0;
+    double best_ms = 0.0;
+    try {
+      for (size_t q = 0; q < cand.size(); ++q) {
+        slot = cand[q];
+        sbdx::sigma_block(*c, prec, k, x.get(), y.get(), c->stream);  // builds this variant's tiles
+        SBDX_HIP(hipEventRecord(e0, c->stream));
+        for (int r = 0; r < reps; ++r) sbdx::sigma_block(*c, prec, k, x.get(), y.get(), c->stream);
+        SBDX_HIP(hipEventRecord(e1, c->stream));
+        SBDX_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        SBDX_HIP(hipEventElapsedTime(&t, e0, e1));
+        const double mean = static_cast<double>(t) / reps;
+        if (ms && static_cast<int>(q) < cap) ms[q] = mean;
+        if (candidates && static_cast<int>(q) < cap) candidates[q] = cand[q];
+        if (best == 0 || mean < best_ms) {
+          best = cand[q];
+          best_ms = mean;
+        }
+      }
+    } catch (...) {
+      slot = 0;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    slot = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (chosen) *chosen = best;
+    if (ncand) *ncand = static_cast<int>(cand.size());
+  });
+}
+
+int sbdx_batch_create(int count, sbdx_ctx* const* ctxs, sbdx_batch** out) {
+  return guarded([&] {
+    need(out != nullptr && ctxs != nullptr && count >= 1, "bad argument");
+    *out = nullptr;
+    int dev = 0;
+    SBDX_HIP(hipGetDevice(&dev));
+    for (int i = 0; i < count; ++i) {
+      need_ints(ctxs[i]);
+      need(ctxs[i]->device == dev, "every context of a batch must live on the current device");
+      for (int j = 0; j < i; ++j) need(ctxs[j] != ctxs[i], "a context appears twice in the batch");
+    }
+    auto* b = new sbdx_batch();
+    b->ctx.assign(ctxs, ctxs + count);
+    b->device = dev;
+    if (hipStreamCreate(&b->stream) != hipSuccess) {
+      delete b;
+      throw sbdx::Error(SBDX_ERR_DEVICE, "stream creation failed");
+    }
+    *out = b;
+  });
+}
+
+int sbdx_batch_davidson(sbdx_batch* b, const sbdx_davidson_opts* o, double* const* W, sbdx_davidson_result* res) {
+  return guarded([&] {
+    need(b != nullptr && o != nullptr && res != nullptr, "null argument");
+    need(o->max_iteration >= 1 && o->num_block >= 2 && o->num_block <= 24, "num_block must be 2..24");
+    need(o->nroots == 1 && o->block <= 1 && o->preconditioner == 0,
+         "a batch solves the lowest root with one vector per block and the diagonal preconditioner");
+    need(o->precision == SBDX_FP64 || o->precision == SBDX_FP32, "bad precision");
+    sbdx::batch_davidson(*b, *o, W, res);
+  });
+}
+
+void sbdx_batch_destroy(sbdx_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
 int sbdx_profile(sbdx_ctx* c, int enable) {
   return guarded([&] {
     need(c != nullptr, "null context");

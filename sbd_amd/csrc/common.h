@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "sbdx.h"
+
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
@@ -130,10 +132,18 @@ struct sbdx_ctx {
   sbdx::DevBuf<float> hii32;
   sbdx::Workspace ws;
   sbdx::Prof prof;
+  int mix_ept_tuned[2][4] = {};    // autotuned mixed-pass entries per thread [FP64|FP32][K slot]
   bool have_mlp = false;           // learned preconditioner set (sbdx_mlp_set)
   sbdx::DevBuf<float> mlp_w;       // its parameters, rounded to bfloat16
   sbdx::DevBuf<float> mlp_pa, mlp_pb;  // per-string first-layer sums (alpha includes b1)
   int64_t ndet() const { return na * nb; }
+};
+
+// a group of contexts on one device solved together (batch.hip)
+struct sbdx_batch {
+  std::vector<sbdx_ctx*> ctx;
+  int device = 0;
+  hipStream_t stream = nullptr;
 };
 
 namespace sbdx {
@@ -198,18 +208,24 @@ __host__ __device__ inline int det_sign(uint64_t a, uint64_t b) {
 
 __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// index of a block width 1, 2, 4, 8
+inline int k_slot(int k) { return k == 1 ? 0 : k == 2 ? 1 : k == 4 ? 2 : 3; }
+
 // ----------------------------------------------------------------------------- launchers
 
 void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta);  // tables.hip
 void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
 void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
 void reset_mix_tiles(sbdx_ctx& c);              // sigma.hip: drop cached mixed-pass tiles
+std::vector<int> mix_ept_candidates(int k);     // sigma.hip: autotuner search space
 // sigma of nvec vectors stored one after another (vector-major)
 void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 // fused sigma of a block of k vectors stored interleaved, element [det * k + v], k in {1,2,4,8}
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
 void apply_sign(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 void apply_sign_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
+// batched lockstep Davidson of many small problems (batch.hip)
+void batch_davidson(sbdx_batch& b, const sbdx_davidson_opts& o, double* const* W, sbdx_davidson_result* res);
 // learned preconditioner (mlp.hip): T[det*k+v] = R/(theta_v - H_ii) (1 + m), P = float|double
 void mlp_precondition(sbdx_ctx& c, int prec, int k, const double* R, const double* theta,
                       const double* rms, void* T, hipStream_t s);

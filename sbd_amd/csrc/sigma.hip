@@ -22,6 +22,8 @@
 // transformations, i.e. Ogita-Rump-Oishi Dot2 per output element).
 #include "common.h"
 
+#include <vector>
+
 #include <algorithm>
 #include <cstdlib>
 #include <utility>
@@ -344,9 +346,12 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
   stk<T, K>(y + k, a);
 }
 
-// entries per thread of the mixed kernel for a block width K (register budget of the
-// per-entry accumulators); K = 1 may be overridden by SBDX_MIX_EPT
-int mix_ept(int K) {
+// entries per thread of the mixed kernel for a block width K: the context's autotuned choice
+// when there is one (sbdx_autotune), else the default of the per-entry accumulator register
+// budget; for K = 1 the default may be overridden by SBDX_MIX_EPT
+int mix_ept(const sbdx_ctx& c, int prec, int K) {
+  const int tuned = c.mix_ept_tuned[prec == 0 ? 0 : 1][k_slot(K)];
+  if (tuned) return tuned;
   if (K == 1) {
     const char* env = std::getenv("SBDX_MIX_EPT");
     const int want = env ? std::atoi(env) : 16;
@@ -426,7 +431,7 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int ept) {
 
 template <typename T, int K>
 void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
-  const int ept = mix_ept(K);
+  const int ept = mix_ept(c, std::is_same<T, double>::value ? 0 : 1, K);
   const MixTiles& mt = mix_tiles(c, ept);
   if (mt.ntile == 0) return;
   const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
@@ -441,14 +446,24 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
     SBDX_CHECK_LAUNCH();
   };
   const bool small_v = static_cast<int>(c.L) * c.L <= 8 * kMixThreads;
+  auto pick = [&](auto e) {
+    constexpr int E = decltype(e)::value;
+    small_v ? launch(&k_sigma_mixed<T, E, 8, K>) : launch(&k_sigma_mixed<T, E, 16, K>);
+  };
+  using std::integral_constant;
   if constexpr (K == 1) {
-    if (ept == 8) small_v ? launch(&k_sigma_mixed<T, 8, 8, 1>) : launch(&k_sigma_mixed<T, 8, 16, 1>);
-    else if (ept == 16) small_v ? launch(&k_sigma_mixed<T, 16, 8, 1>) : launch(&k_sigma_mixed<T, 16, 16, 1>);
-    else small_v ? launch(&k_sigma_mixed<T, 32, 8, 1>) : launch(&k_sigma_mixed<T, 32, 16, 1>);
+    if (ept == 8) pick(integral_constant<int, 8>{});
+    else if (ept == 16) pick(integral_constant<int, 16>{});
+    else pick(integral_constant<int, 32>{});
   } else if constexpr (K == 2) {
-    small_v ? launch(&k_sigma_mixed<T, 8, 8, 2>) : launch(&k_sigma_mixed<T, 8, 16, 2>);
+    if (ept == 4) pick(integral_constant<int, 4>{});
+    else if (ept == 16) pick(integral_constant<int, 16>{});
+    else pick(integral_constant<int, 8>{});
+  } else if constexpr (K == 4) {
+    if (ept == 8) pick(integral_constant<int, 8>{});
+    else pick(integral_constant<int, 4>{});
   } else {
-    small_v ? launch(&k_sigma_mixed<T, 4, 8, K>) : launch(&k_sigma_mixed<T, 4, 16, K>);
+    pick(integral_constant<int, 4>{});
   }
 }
 
@@ -536,6 +551,14 @@ void sign_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
 
 void reset_mix_tiles(sbdx_ctx& c) {
   for (auto& m : c.mix) m = MixTiles{};
+}
+
+// entries-per-thread candidates of the mixed kernel for block width K
+std::vector<int> mix_ept_candidates(int K) {
+  if (K == 1) return {8, 16, 32};
+  if (K == 2) return {4, 8, 16};
+  if (K == 4) return {4, 8};
+  return {4};
 }
 
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {

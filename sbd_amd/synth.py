@@ -222,3 +222,25 @@ def make_custom(norb: int, n_alpha_el: int, n_beta_el: int, rank: int,
     return Instance(name=name, ints=ints, alpha=alpha, beta=beta, n_alpha_el=n_alpha_el,
                     n_beta_el=n_beta_el, meta=dict(seed_ints=seed_ints, seed_alpha=seed_alpha,
                                                    seed_beta=seed_beta, rank=rank))
+
+
+def make_batch_instances(count: int = 64, seed: int = 0, dets=(1e3, 1e6)) -> list:
+    """Many small problems of the same recipe: norb uniform in {10..20}, norb // 3 electrons per
+    spin, rank-(2 norb) integrals with a seed of their own, and Fermi-sampled strings (equal
+    counts per spin) giving a log-uniform determinant count in [dets[0], dets[1]]; where that
+    needs more than half of a spin's string space, the full space is used instead (weighted
+    sampling of nearly every string would wait on the rarest ones)."""
+    from math import comb
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        L = int(rng.integers(10, 21))
+        ne = L // 3
+        target = float(np.exp(rng.uniform(np.log(dets[0]), np.log(dets[1]))))
+        full = comb(L, ne)
+        ns = max(2, round(np.sqrt(target)))
+        n = None if 2 * ns > full else ns
+        s = 1000 + 3 * (seed * count + k)
+        out.append(make_custom(L, ne, ne, 2 * L, n, n, seed_ints=s, seed_alpha=s + 1,
+                               seed_beta=s + 2, name=f"batch{seed}_{k}"))
+    return out
