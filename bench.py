@@ -46,6 +46,12 @@ SOLVERS = {
     "mixed": dict(method=1, precision=1, block=1, num_block=10),
     "mixed_block2": dict(method=1, precision=1, block=2, num_block=8),
     "mixed_block4": dict(method=1, precision=1, block=4, num_block=6),
+    "mixed_block2_nb12": dict(method=1, precision=1, block=2, num_block=12),
+    "mixed_block2_nb16": dict(method=1, precision=1, block=2, num_block=16),
+    "block2_fp64_nb16": dict(method=1, precision=0, block=2, num_block=16),
+    # the learned preconditioner trained on 36-orbital family instances (sbd_amd/mlp_train.py)
+    "mixed_block2_nb16_mlp": dict(method=1, precision=1, block=2, num_block=16, preconditioner=1,
+                                  mlp="sbd_amd/weights/c2_family.npy"),
 }
 
 
@@ -122,6 +128,39 @@ def cpu_baseline(inst_family="c2", strings=1000, threads=None):
     return dict(seconds=dt, nnz=nnz, ndet=inst.ndet, kind="port", cores=threads, sample=sample)
 
 
+def batch_bench(count, prec):
+    """Many small problems (norb 10..20, 1e3..1e6 determinants, log-uniform) solved to 1e-8
+    one after another with the block solver, then together with the batched graph-replayed
+    solver; energies must agree to 1e-9 Ha."""
+    import torch
+    from sbd_amd import synth
+    from sbd_amd._lib import Batch, Engine
+    insts = synth.make_batch_instances(count, seed=0)
+    engs = [Engine.from_instance(i) for i in insts]
+    opts = dict(num_block=10, max_iteration=2000, eps=1e-8, precision=prec)
+    for e in engs:  # warm every problem's tables and kernels
+        e.davidson(method=1, max_iteration=1, num_block=2, eps=0.0, precision=prec)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    single = [e.davidson(method=1, **opts) for e in engs]
+    t_seq = time.perf_counter() - t
+    b = Batch(engs)
+    b.davidson(**dict(opts, max_iteration=1))  # capture the step graphs once
+    t = time.perf_counter()
+    res = b.davidson(**opts)
+    t_bat = time.perf_counter() - t
+    b.close()
+    diff = max(abs(r["energy"] - s["energy"]) for r, s in zip(res, single))
+    rec = dict(count=count, dets_total=int(sum(i.ndet for i in insts)),
+               dets_min=int(min(i.ndet for i in insts)), dets_max=int(max(i.ndet for i in insts)),
+               precision="fp32" if prec else "fp64", sequential_s=round(t_seq, 3),
+               batched_s=round(t_bat, 3), speedup=round(t_seq / t_bat, 3),
+               converged=int(sum(r["converged"] for r in res)),
+               max_energy_diff_vs_single=diff)
+    log(f"[bench] batch: {rec}")
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +176,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain 1 GiB device copy (rocprof traffic calibration)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="also solve this many small problems one by one and as one batch")
+    ap.add_argument("--batch-prec", type=int, default=1, choices=[0, 1])
     ap.add_argument("--autotune", action="store_true",
                     help="autotune the mixed pass of each --block width before timing it")
     ap.add_argument("--block", type=int, nargs="*", default=[],
@@ -305,6 +347,8 @@ def main():
         for name in names:
             opts = dict(SOLVERS[name], max_iteration=100000, eps=1e-8, max_time=args.solve_time,
                         verbose=int(args.verbose))
+            weights = opts.pop("mlp", None)
+            eng.mlp_set(np.load(os.path.join(ROOT, weights)) if weights else None)
             if e_ref is not None:
                 opts.update(energy_ref=e_ref, energy_tol=1e-6, stop_at_tol=1)
             k = max(1, opts.get("nroots", 1))
@@ -324,6 +368,9 @@ def main():
         if "plain" in t and t["plain"] > 0:
             out["solve"]["speedup_vs_plain"] = {n: round(t["plain"] / v, 3) for n, v in t.items()
                                                 if v > 0 and n != "plain"}
+
+    if args.batch:
+        out["batch"] = batch_bench(args.batch, args.batch_prec)
 
     if not args.no_cpu_baseline:
         cb = cpu_baseline(args.workload if args.workload in ("c2", "c3") else "c2")

@@ -98,8 +98,8 @@ typedef struct {
                              correction blocks and sigmas, FP64 anchor sigma, Rayleigh-Ritz,
                              orthogonalization and residual */
   int nroots;             /* lowest roots returned and converged, 1..8 (> 1: method 1) */
-  int preconditioner;     /* 0: diagonal (Jacobi); 1: learned (method 1; falls back to 0 on
-                             stall) */
+  int preconditioner;     /* 0: diagonal (Jacobi); 1: learned (method 1; falls back to 0 for
+                             good when a restart cycle fails to halve the exact residual) */
   int verbose;            /* print the reference's per-step line */
   double energy_ref;      /* if nonzero: record time to |E0 - energy_ref| < energy_tol, judged
                              on FP64 Ritz values only */
@@ -109,6 +109,10 @@ typedef struct {
   int block;              /* method 1: vectors per block (1, 2, 4, 8; rounded up to a power of
                              two >= nroots); 0 = nroots */
   int stop_at_tol;        /* 1: stop once energy_ref is reached (time-to-solution runs) */
+  int precond_warmup;     /* preconditioner 1: restart cycles run with the diagonal
+                             preconditioner before the learned one takes over (default 1:
+                             the first cycle, from the start vector, far from the training
+                             distribution of Ritz values) */
 } sbdx_davidson_opts;
 
 typedef struct {

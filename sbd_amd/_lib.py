@@ -36,7 +36,7 @@ class DavidsonOpts(C.Structure):
                 ("max_time", C.c_double), ("init", C.c_int), ("precision", C.c_int),
                 ("nroots", C.c_int), ("preconditioner", C.c_int), ("verbose", C.c_int),
                 ("energy_ref", C.c_double), ("energy_tol", C.c_double), ("method", C.c_int),
-                ("block", C.c_int), ("stop_at_tol", C.c_int)]
+                ("block", C.c_int), ("stop_at_tol", C.c_int), ("precond_warmup", C.c_int)]
 
 
 class DavidsonResult(C.Structure):

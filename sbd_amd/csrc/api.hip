@@ -233,6 +233,7 @@ void sbdx_davidson_defaults(sbdx_davidson_opts* o) {
   o->method = 0;
   o->block = 0;
   o->stop_at_tol = 0;
+  o->precond_warmup = 1;
 }
 
 // validates the options; returns the vectors per block of the device work buffer
@@ -245,6 +246,7 @@ static int check_opts(const sbdx_ctx* c, const sbdx_davidson_opts* o) {
   need(o->preconditioner == 0 || c->have_mlp, "preconditioner 1 needs a network (sbdx_mlp_set)");
   need(o->method == 0 || o->method == 1, "method must be 0 or 1");
   need(o->block >= 0 && o->block <= 8, "block must be 0..8");
+  need(o->precond_warmup >= 0, "precond_warmup must be >= 0");
   if (o->method == 0) {
     need(o->nroots == 1 && o->block <= 1 && o->precision == SBDX_FP64 && o->preconditioner == 0,
          "method 0 (the reference algorithm) is FP64, one root, diagonal preconditioner; use method 1");

@@ -12,8 +12,8 @@
 // share the GPU inside one launch.
 //
 // Batched kernels take grid (workgroups per instance, instances); instance i grid-strides over
-// its own n_i, and an instance that has converged is skipped by its workgroups (and left out of
-// the next graph).
+// its own n_i, and an instance that has converged is skipped by its workgroups at once (a device
+// flag) and left out of the sigma builds of the graphs captured after it.
 #include "common.h"
 #include "sbdx.h"
 
@@ -365,13 +365,6 @@ struct BatchSolver {
         SBDX_HIP(hipMemcpyAsync(h_res.data() + i * kSlots, res[i].get(), kSlots * 8, hipMemcpyDeviceToHost, st));
     SBDX_HIP(hipStreamSynchronize(st));
   }
-  void set_active(int i, int v) {
-    active[i] = v;
-    SBDX_HIP(hipMemcpyAsync(dactive.get() + i, &active[i], sizeof(int), hipMemcpyHostToDevice, st));
-    SBDX_HIP(hipStreamSynchronize(st));
-    ++epoch;
-  }
-
   // anchor phase: sigma of every active X in FP64 and its Rayleigh data
   void anchor_phase() {
     graph(1000, [&] {
@@ -458,7 +451,7 @@ struct BatchSolver {
     }
     std::vector<double> E(cnt, 0.0), nrm(cnt, 0.0);
     std::vector<int> steps(cnt, 0);
-    int left = cnt;
+    int left = cnt, captured = cnt;
     for (int it = 0; it < o.max_iteration && left > 0; ++it) {
       anchor_phase();
       int nb = 1;
@@ -505,7 +498,14 @@ struct BatchSolver {
         if (changed) {
           SBDX_HIP(hipMemcpyAsync(dactive.get(), active.data(), cnt * sizeof(int), hipMemcpyHostToDevice, st));
           SBDX_HIP(hipStreamSynchronize(st));
-          ++epoch;
+          // converged problems drop out of the batched kernels at once (device flags); the
+          // graphs, whose sigma builds still include them, are recaptured only when a quarter
+          // of their problems have left, which bounds both the wasted sigma work and the
+          // number of captures
+          if (4 * left <= 3 * captured) {
+            ++epoch;
+            captured = left;
+          }
         }
         if (left == 0 || now() - t0 > o.max_time) break;
         if (last) break;

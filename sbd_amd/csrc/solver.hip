@@ -9,7 +9,10 @@
 // blocks, by canonical orthogonalization, so FP32 blocks need not be exactly orthonormal.  New
 // blocks: preconditioned residuals, two passes of classical Gram-Schmidt against the subspace in
 // FP64 arithmetic, then Loewdin orthonormalization of the block itself.  Thick restart keeps the
-// K Ritz vectors of the whole subspace as the next anchor.
+// K Ritz vectors of the whole subspace as the next anchor, and the correction part of those
+// Ritz vectors (their component outside the old anchor, with its sigma, a linear combination of
+// the correction blocks' sigmas) as the first block of the next subspace: the restarted method
+// then keeps the previous search direction, as LOBPCG does, instead of losing it.
 //
 // Every block product is tall-skinny: the kernels stream their blocks once ([N][K] interleaved,
 // K contiguous values per determinant) and reduce K x K partial sums per workgroup.
@@ -233,6 +236,74 @@ __global__ __launch_bounds__(kB) void k_project_out(P* __restrict__ T, const dou
   }
 }
 
+// The correction part of the Ritz block and its sigma: P = sum_{i>=1} B_i u_i,
+// HP = sum_{i>=1} HB_i u_i (FP64 sums, stored in P); the search direction kept across a restart
+template <typename P, int K>
+__global__ __launch_bounds__(kB) void k_corr(BlockPtrs bp, int nblocks, const double* __restrict__ coef,
+                                             P* __restrict__ outP, P* __restrict__ outHP, int64_t n) {
+  __shared__ double cs[kMaxBlocks * K * K];
+  for (int q = threadIdx.x; q < nblocks * K * K; q += kB) cs[q] = coef[q];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    double y[K], hy[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) y[v] = hy[v] = 0.0;
+    for (int blk = 1; blk < nblocks; ++blk) {
+      const P* B = static_cast<const P*>(bp.b[blk]);
+      const P* HB = static_cast<const P*>(bp.hb[blk]);
+      const double* cb = cs + blk * K * K;
+#pragma unroll
+      for (int a = 0; a < K; ++a) {
+        const double xa = static_cast<double>(B[i * K + a]), ha = static_cast<double>(HB[i * K + a]);
+#pragma unroll
+        for (int v = 0; v < K; ++v) {
+          y[v] = __fma_rn(xa, cb[a * K + v], y[v]);
+          hy[v] = __fma_rn(ha, cb[a * K + v], hy[v]);
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < K; ++v) {
+      outP[i * K + v] = static_cast<P>(y[v]);
+      outHP[i * K + v] = static_cast<P>(hy[v]);
+    }
+  }
+}
+
+// P -= X c, HP -= HX c (c: K x K row-major [a][v]); FP64 arithmetic
+template <typename P, int K>
+__global__ __launch_bounds__(kB) void k_project_pair(P* __restrict__ Pv, P* __restrict__ HPv, const double* __restrict__ X,
+                                                     const double* __restrict__ HX, const double* __restrict__ coef,
+                                                     int64_t n) {
+  __shared__ double cs[K * K];
+  for (int q = threadIdx.x; q < K * K; q += kB) cs[q] = coef[q];
+  __syncthreads();
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    double p[K], hp[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) {
+      p[v] = static_cast<double>(Pv[i * K + v]);
+      hp[v] = static_cast<double>(HPv[i * K + v]);
+    }
+#pragma unroll
+    for (int a = 0; a < K; ++a) {
+      const double xa = X[i * K + a], ha = HX[i * K + a];
+#pragma unroll
+      for (int v = 0; v < K; ++v) {
+        p[v] = __fma_rn(-xa, cs[a * K + v], p[v]);
+        hp[v] = __fma_rn(-ha, cs[a * K + v], hp[v]);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < K; ++v) {
+      Pv[i * K + v] = static_cast<P>(p[v]);
+      HPv[i * K + v] = static_cast<P>(hp[v]);
+    }
+  }
+}
+
 // Y <- Y M (K x K, row-major [a][v]) in place
 template <typename TY, int K>
 __global__ __launch_bounds__(kB) void k_rotate(TY* __restrict__ Y, const double* __restrict__ M, int64_t n) {
@@ -342,9 +413,12 @@ struct BlockSolver {
   DevBuf<double> X, HX, R;          // anchor, its FP64 sigma, residual / next anchor
   std::vector<DevBuf<P>> B, HB;     // corrections 1..m-1 and their sigmas (B[0] unused)
   DevBuf<P> T;                      // block under construction
+  DevBuf<P> Pd, HPd;                // search direction kept across a restart, and its sigma
+  bool have_dir = false;
   DevBuf<double> partial, small;    // reduction partials; staged coefficients
   std::vector<double> Hs, Ss;       // (m K) x (m K) subspace matrices, row-major
   bool use_mlp;                     // learned preconditioner in use
+  int cycle = 0;                    // restart cycle of the current step
   int fallback_step = 0;            // step at which it was dropped (0: never)
   static constexpr int prec = std::is_same<P, float>::value ? SBDX_FP32 : SBDX_FP64;
 
@@ -356,6 +430,8 @@ struct BlockSolver {
     HX.alloc(n * K);
     R.alloc(n * K);
     T.alloc(n * K);
+    Pd.alloc(n * K);
+    HPd.alloc(n * K);
     B.resize(m);
     HB.resize(m);
     for (int i = 1; i < m; ++i) {
@@ -512,7 +588,7 @@ struct BlockSolver {
   // preconditioned residuals in T: the learned preconditioner while it is in use (falling
   // back for good if it yields a non-finite block), else the diagonal one
   void precondition(const std::vector<double>& E, const std::vector<double>& nr, int nstep) {
-    if (use_mlp) {
+    if (use_mlp && cycle >= o.precond_warmup) {
       std::vector<double> rms(K);
       for (int v = 0; v < K; ++v) rms[v] = nr[v] / std::sqrt(static_cast<double>(n));
       mlp_precondition(c, prec, K, R.get(), E.data(), rms.data(), T.get(), st);
@@ -541,6 +617,33 @@ struct BlockSolver {
       if (pass == 0) loewdin(T.get());  // rescale before the second pass
     }
     loewdin(T.get());
+  }
+
+  // the kept direction, orthogonalised against the new anchor (its sigma updated with the exact
+  // FP64 sigma of the anchor) and orthonormalised, becomes block 1 of the new subspace
+  void adopt_direction() {
+    std::vector<double> cxp(K * K), G(K * K), Gc(K * K), w(K), M(K * K, 0.0);
+    gram(X.get(), Pd.get(), cxp.data());
+    upload(small.get(), cxp.data(), cxp.size());
+    k_project_pair<P, K><<<nblk(n), kB, 0, st>>>(Pd.get(), HPd.get(), X.get(), HX.get(), small.get(), n);
+    SBDX_CHECK_LAUNCH();
+    gram(Pd.get(), Pd.get(), G.data());
+    for (int a = 0; a < K; ++a)
+      for (int b = 0; b < K; ++b) Gc[a + K * b] = G[a * K + b];
+    jacobi_syev_host(K, Gc.data(), K, w.data());
+    const double lmax = std::max(w[K - 1], 1e-300);
+    for (int a = 0; a < K; ++a)
+      for (int v = 0; v < K; ++v) M[a * K + v] = w[v] > 1e-14 * lmax ? Gc[a + K * v] / std::sqrt(w[v]) : 0.0;
+    upload(small.get(), M.data(), M.size());
+    k_rotate<P, K><<<nblk(n), kB, 0, st>>>(Pd.get(), small.get(), n);
+    SBDX_CHECK_LAUNCH();
+    k_rotate<P, K><<<nblk(n), kB, 0, st>>>(HPd.get(), small.get(), n);
+    SBDX_CHECK_LAUNCH();
+    std::swap(B[1], Pd);
+    std::swap(HB[1], HPd);
+    fill_block(0, 1);
+    fill_block(1, 1);
+    have_dir = false;
   }
 
   void initial_block(const double* dW0) {
@@ -581,11 +684,12 @@ struct BlockSolver {
     res.seconds_to_tol = -1.0;
     res.steps_to_tol = -1;
     initial_block(dW);
-    int nstep = 0, stalls = 0;
+    int nstep = 0;
     bool conv = false, stop = false;
-    double best_anchor = 0.0;
+    double prev_anchor = 0.0;
     std::vector<double> U, E(K, 0.0), nr(K, 0.0), ny;
     for (int it = 0; it < o.max_iteration && !stop; ++it) {
+      cycle = it;
       // anchor: exact FP64 sigma of X
       sigma_block(c, SBDX_FP64, K, X.get(), HX.get(), st);
       std::fill(Hs.begin(), Hs.end(), 0.0);
@@ -612,15 +716,12 @@ struct BlockSolver {
         const bool near_ref = o.energy_ref != 0.0 && std::fabs(E[0] - o.energy_ref) < o.energy_tol;
         if (exact) {
           // stall test of the learned preconditioner on the exact residual of every restart:
-          // two restarts in a row without a new lowest residual switch to the diagonal one
-          if (use_mlp && it > 0) {
-            stalls = worst < best_anchor ? 0 : stalls + 1;
-            if (stalls >= 2) {
-              use_mlp = false;
-              fallback_step = nstep;
-            }
+          // a cycle that does not halve it switches to the diagonal preconditioner for good
+          if (use_mlp && it > o.precond_warmup && worst > 0.5 * prev_anchor) {
+            use_mlp = false;
+            fallback_step = nstep;
           }
-          if (it == 0 || worst < best_anchor) best_anchor = worst;
+          prev_anchor = worst;
           if (near_ref && res.seconds_to_tol < 0) {
             res.seconds_to_tol = now() - t0;
             res.steps_to_tol = nstep;
@@ -635,14 +736,30 @@ struct BlockSolver {
         // a mixed-precision estimate that meets a criterion is confirmed by a restart
         if (!exact && (worst < o.eps || (near_ref && res.seconds_to_tol < 0))) break;
         if (nblocks == m) break;
+        if (exact && have_dir && m > 2) {
+          adopt_direction();
+          nblocks = 2;
+          continue;
+        }
         expand(nblocks, E, nr, nstep);
         std::swap(B[nblocks], T);
         sigma_block(c, prec, K, B[nblocks].get(), HB[nblocks].get(), st);
         for (int i = 0; i <= nblocks; ++i) fill_block(i, nblocks);
         ++nblocks;
       }
-      // thick restart (or the final vectors): X <- normalised Ritz vectors of the subspace
+      // thick restart (or the final vectors): X <- normalised Ritz vectors of the subspace; the
+      // correction part of the Ritz block is kept as the next cycle's first search direction
       combine(nblocks, U, E, 1, ny);
+      if (nblocks > 1 && !stop) {
+        std::vector<double> stage(nblocks * K * K);
+        for (int i = 0; i < nblocks; ++i)
+          for (int a = 0; a < K; ++a)
+            for (int v = 0; v < K; ++v) stage[i * K * K + a * K + v] = U[(i * K + a) * K + v];
+        upload(small.get(), stage.data(), stage.size());
+        k_corr<P, K><<<nblk(n), kB, 0, st>>>(ptrs(nblocks), nblocks, small.get(), Pd.get(), HPd.get(), n);
+        SBDX_CHECK_LAUNCH();
+        have_dir = true;
+      }
       std::vector<double> M(K * K, 0.0);
       for (int v = 0; v < K; ++v) M[v * K + v] = ny[v] > 0 ? 1.0 / ny[v] : 0.0;
       upload(small.get(), M.data(), M.size());

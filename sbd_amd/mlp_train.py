@@ -114,6 +114,8 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
+    import torch
+    torch.zeros(1, device="cuda")  # device memory comes from torch: initialise it first
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)

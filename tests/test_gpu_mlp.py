@@ -113,15 +113,16 @@ def test_record_matches_oracle():
 
 def test_zero_last_layer_is_the_diagonal_preconditioner():
     """A network with w3 = b3 = 0 outputs m = 0: preconditioner 1 returns r / (theta - H_ii)
-    bit for bit, and the solver follows the diagonal preconditioner's iteration (to the
+    to the last bit or two, and the solver follows the diagonal preconditioner's iteration (to the
     rounding of the FP32 sigma, whose mixed pass folds entries with LDS atomics in no fixed
     order) to the same energy."""
     inst, so, eng = small_case()
-    hii = so.hdiag()
+    hii = eng.hdiag()  # the engine's own diagonal (the oracle's agrees to 1e-12, not bitwise)
     r = np.random.default_rng(8).standard_normal(so.ndet)
     theta = hii.min() - 0.25
     eng.mlp_set(M.init_params(5, zero_last=True))
-    assert np.array_equal(eng.mlp_apply(theta, r), r / (theta - hii))
+    ref = r / (theta - hii)
+    assert np.max(np.abs(eng.mlp_apply(theta, r) - ref) / np.abs(ref)) <= 4e-16
     a = eng.davidson(method=1, precision=1, num_block=6, max_iteration=60, eps=1e-9, preconditioner=0)
     b = eng.davidson(method=1, precision=1, num_block=6, max_iteration=60, eps=1e-9, preconditioner=1)
     assert a["converged"] and b["converged"] and b["precond_fallback"] == 0
