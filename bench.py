@@ -52,6 +52,8 @@ SOLVERS = {
     # the learned preconditioner trained on 36-orbital family instances (sbd_amd/mlp_train.py)
     "mixed_block2_nb16_mlp": dict(method=1, precision=1, block=2, num_block=16, preconditioner=1,
                                   mlp="sbd_amd/weights/c2_family.npy"),
+    # eight lowest roots in one block of eight
+    "mixed_roots8": dict(method=1, precision=1, nroots=8, num_block=4),
 }
 
 
@@ -128,6 +130,39 @@ def cpu_baseline(inst_family="c2", strings=1000, threads=None):
     return dict(seconds=dt, nnz=nnz, ndet=inst.ndet, kind="port", cores=threads, sample=sample)
 
 
+def precision_report(eng, max_time):
+    """The workload solved to residual 1e-8 twice -- the reference algorithm in FP64 and the
+    mixed-precision block solver (FP32 corrections, FP64 anchor refinement) -- and the errors of
+    the mixed run's energy, 1-RDM and 2-RDM against the FP64 run."""
+    import torch
+    N = eng.ndet
+    out = {}
+    vecs = {}
+    for name, opts in (("fp64_reference_algorithm", dict(method=0, num_block=10)),
+                       ("mixed_block2_nb16", SOLVERS["mixed_block2_nb16"])):
+        W = torch.zeros(N * 2, dtype=torch.float64, device="cuda")
+        t = time.perf_counter()
+        r = eng.davidson_device(W.data_ptr(), **dict(opts, max_iteration=100000, eps=1e-8,
+                                                     max_time=max_time))
+        dt = time.perf_counter() - t
+        w = W[:N].cpu().numpy()
+        t = time.perf_counter()
+        one, two = eng.rdm(w)
+        t_rdm = time.perf_counter() - t
+        vecs[name] = (w, one, two)
+        out[name] = dict(energy=r["energy"], residual=r["residual"], converged=r["converged"],
+                         steps=r["steps"], seconds=round(dt, 3), rdm_seconds=round(t_rdm, 3))
+        log(f"[bench] precision report {name}: {out[name]}")
+        del W
+        torch.cuda.empty_cache()
+    (w0, o0, t0_), (w1, o1, t1_) = vecs["fp64_reference_algorithm"], vecs["mixed_block2_nb16"]
+    out["energy_error"] = abs(out["mixed_block2_nb16"]["energy"] - out["fp64_reference_algorithm"]["energy"])
+    out["one_rdm_max_error"] = float(np.max(np.abs(o1 - o0)))
+    out["two_rdm_max_error"] = float(np.max(np.abs(t1_ - t0_)))
+    out["vector_overlap"] = float(abs(w0 @ w1))
+    return out
+
+
 def batch_bench(count, prec):
     """Many small problems (norb 10..20, 1e3..1e6 determinants, log-uniform) solved to 1e-8
     one after another with the block solver, then together with the batched graph-replayed
@@ -172,10 +207,14 @@ def main():
     ap.add_argument("--solve", nargs="*", default=None, choices=sorted(SOLVERS),
                     help="also time these Davidson variants to 1e-6 Ha of the reference energy")
     ap.add_argument("--solve-time", type=float, default=240.0)
+    ap.add_argument("--converge", action="store_true",
+                    help="--solve runs to residual 1e-8 instead of stopping at the reference energy")
     ap.add_argument("--verbose", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one plain 1 GiB device copy (rocprof traffic calibration)")
+    ap.add_argument("--precision-report", action="store_true",
+                    help="solve to 1e-8 in FP64 and in mixed precision; energy and RDM errors")
     ap.add_argument("--batch", type=int, default=0,
                     help="also solve this many small problems one by one and as one batch")
     ap.add_argument("--batch-prec", type=int, default=1, choices=[0, 1])
@@ -342,7 +381,7 @@ def main():
         names = args.solve or ["plain", "mixed_block2"]
         del x, y
         torch.cuda.empty_cache()
-        e_ref = reference_energy(args.workload, args.strings)
+        e_ref = None if args.converge else reference_energy(args.workload, args.strings)
         out["solve"] = {"energy_ref": e_ref, "energy_tol": 1e-6}
         for name in names:
             opts = dict(SOLVERS[name], max_iteration=100000, eps=1e-8, max_time=args.solve_time,
@@ -368,6 +407,9 @@ def main():
         if "plain" in t and t["plain"] > 0:
             out["solve"]["speedup_vs_plain"] = {n: round(t["plain"] / v, 3) for n, v in t.items()
                                                 if v > 0 and n != "plain"}
+
+    if args.precision_report:
+        out["precision_report"] = precision_report(eng, args.solve_time)
 
     if args.batch:
         out["batch"] = batch_bench(args.batch, args.batch_prec)

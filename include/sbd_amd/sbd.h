@@ -268,6 +268,20 @@ struct Engine {
 
 }  // namespace amd_detail
 
+// Solver used by the drop-in Davidson below; no counterpart in the reference, whose algorithm
+// the defaults keep.  generate_sbd_data also reads --amd_solver block|reference,
+// --amd_precision fp32|fp64, --amd_block k and --amd_num_block m into it.
+struct AmdSolver {
+  int method = 0;      // 0: the reference's algorithm; 1: block thick-restart Davidson
+  int precision = 0;   // SBDX_FP64, or SBDX_FP32 (mixed precision, method 1)
+  int block = 0;       // vectors per block (method 1)
+  int num_block = 0;   // blocks per subspace (0: the caller's num_block)
+};
+inline AmdSolver& amd_solver() {
+  static AmdSolver s;
+  return s;
+}
+
 struct TaskHelpers {
   size_t braAlphaStart, braAlphaEnd, ketAlphaStart, ketAlphaEnd;
   size_t braBetaStart, braBetaEnd, ketBetaStart, ketBetaEnd;
@@ -405,6 +419,13 @@ void Davidson(const std::vector<ElemT>& hii, std::vector<ElemT>& W, const std::v
   o.max_time = max_time;
   o.init = 1;
   o.verbose = 1;
+  const AmdSolver& as = amd_solver();
+  if (as.method == 1) {
+    o.method = 1;
+    o.precision = as.precision;
+    o.block = as.block;
+    if (as.num_block > 0) o.num_block = as.num_block;
+  }
   sbdx_davidson_result r;
   amd_detail::check(sbdx_davidson(e.ctx, &o, W.data(), &r, nullptr, nullptr, 0));
 }
@@ -555,6 +576,10 @@ inline SBD generate_sbd_data(int argc, char* argv[]) {
     else if (a == "--shuffle") d.do_shuffle = std::atoi(v);
     else if (a == "--rdm") d.do_rdm = std::atoi(v);
     else if (a == "--bit_length") d.bit_length = std::atoi(v);
+    else if (a == "--amd_solver") amd_solver().method = std::string(v) == "block" ? 1 : 0;
+    else if (a == "--amd_precision") amd_solver().precision = std::string(v) == "fp32" ? SBDX_FP32 : SBDX_FP64;
+    else if (a == "--amd_block") amd_solver().block = std::atoi(v);
+    else if (a == "--amd_num_block") amd_solver().num_block = std::atoi(v);
     else used = false;
     if (used) i++;
   }

@@ -57,7 +57,9 @@ int main(int argc, char** argv) {
   };
   auto adet = words(a), bdet = words(b);
 
-  sbd::tpb::SBD d;
+  // extra arguments after the tolerance are options in the sample driver's form
+  // (e.g. --amd_solver block --amd_precision fp32)
+  sbd::tpb::SBD d = sbd::tpb::generate_sbd_data(argc, argv);
   d.max_it = 200;
   d.max_nb = 10;
   d.eps = argc > 3 ? std::atof(argv[3]) : 1e-8;

@@ -45,3 +45,22 @@ def test_cpp_drop_in_api(tmp_path):
     assert res["carryover"] >= 1
     wf = out + ".wf000000"
     assert os.path.getsize(wf) == 3 * 8 + (60 + 50) * 8 + so.ndet * 8
+
+
+def test_cpp_drop_in_with_the_block_mixed_solver(tmp_path):
+    """The same driver with --amd_solver block --amd_precision fp32: the drop-in's Davidson
+    runs the block mixed-precision solver and reaches the same energy."""
+    if not os.path.exists(BIN):
+        pytest.skip("tests/cpp/shim_diag not built (needs the image's MPI)")
+    inst = synth.make_custom(10, 3, 3, 8, 60, 50, seed_ints=3)
+    vals, idx = inst.ints.fcidump_entries()
+    path = str(tmp_path / "inst.bin")
+    O.write_instance(path, inst.norb, 3, 3, inst.alpha, inst.beta, vals, idx)
+    out = str(tmp_path / "shim")
+    r = subprocess.run([BIN, path, out, "1e-8", "--amd_solver", "block", "--amd_precision", "fp32",
+                        "--amd_block", "2"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    ref = O.System.from_instance(inst).davidson(max_it=200, num_block=10, eps=1e-8)
+    assert abs(res["energy"] - ref["energy"]) < 1e-9
+    assert "(mixed)" in r.stdout
