@@ -304,12 +304,13 @@ void davidson_fp64(sbdx_ctx& c, const sbdx_davidson_opts& o, double* dW, sbdx_da
         res.seconds_to_tol = now() - t0;
         res.steps_to_tol = nstep;
       }
-      const bool last = norm_R < o.eps;
+      const bool at_tol = o.stop_at_tol && res.steps_to_tol == nstep;
+      const bool last = norm_R < o.eps || at_tol;
       double* cnew = (!last && ib < nb - 1) ? C[ib + 1].get() : nullptr;
       k_precond<<<sb, kBlock, 0, st>>>(dW, 1.0 / norm_W, R.get(), 1.0 / norm_R, E[0], c.hii.get(), cnew, n);
       SBDX_CHECK_LAUNCH();
       if (last) {
-        conv = true;
+        conv = norm_R < o.eps;
         cont = false;
         break;
       }

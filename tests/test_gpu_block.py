@@ -92,6 +92,16 @@ def test_time_to_reference_energy(small):
     assert not r["converged"] or r["residual"] < 1e-12
 
 
+def test_time_to_reference_energy_method0(small):
+    """The reference's algorithm stops at the reference energy too, with a normalised vector."""
+    name, so, eng, H, evals = small
+    r = eng.davidson(method=0, num_block=10, max_iteration=300, eps=1e-12,
+                     energy_ref=evals[0], energy_tol=1e-6, stop_at_tol=1)
+    assert r["steps_to_tol"] > 0 and r["steps"] == r["steps_to_tol"]
+    assert abs(r["energy"] - evals[0]) < 1e-6
+    assert abs(np.linalg.norm(r["W"]) - 1.0) < 1e-9
+
+
 def test_method0_rejects_block_options(small):
     name, so, eng, H, evals = small
     with pytest.raises(SbdxError):
