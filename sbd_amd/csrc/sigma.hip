@@ -352,12 +352,15 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
 int mix_ept(const sbdx_ctx& c, int prec, int K) {
   const int tuned = c.mix_ept_tuned[prec == 0 ? 0 : 1][k_slot(K)];
   if (tuned) return tuned;
+  // defaults: the autotuner's choices on the 4e8-determinant 36-orbital workload
   if (K == 1) {
     const char* env = std::getenv("SBDX_MIX_EPT");
-    const int want = env ? std::atoi(env) : 16;
-    return (want == 8 || want == 16 || want == 32) ? want : 16;
+    const int def = prec == 0 ? 16 : 32;
+    const int want = env ? std::atoi(env) : def;
+    return (want == 8 || want == 16 || want == 32) ? want : def;
   }
-  return K == 2 ? 8 : 4;
+  if (K == 2) return prec == 0 ? 16 : 8;
+  return K == 4 ? 8 : 4;
 }
 
 int ept_slot(int ept) { return ept == 4 ? 0 : ept == 8 ? 1 : ept == 16 ? 2 : 3; }
