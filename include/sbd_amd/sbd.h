@@ -26,8 +26,10 @@
 //   SaveWavefunction               chemistry/tpb/restart.h:20-67 (one rank's file)
 //   tpb::SBD, generate_sbd_data,   chemistry/tpb/sbdiag.h:12-85, :102-456
 //   tpb::diag
-// Not provided here: the stored-Hamiltonian method (method 1), LoadWavefunction, and the
-// generic operator layer (include/sbd/basic/); diag throws std::invalid_argument for them.
+//   LoadWavefunction               chemistry/tpb/restart.h:414-657 (one rank's file)
+//   tpb::diag method 1             makeQCham + stored-H mult (qcham.h:10-312, mult.h:13-123)
+//                                  as sbdx_store_hamiltonian
+// Not provided here: the generic operator layer (include/sbd/basic/).
 #ifndef SBD_AMD_SBD_H
 #define SBD_AMD_SBD_H
 
@@ -536,6 +538,61 @@ void SaveWavefunction(const std::string file, const std::vector<std::vector<size
   f.write(reinterpret_cast<const char*>(W.data()), sizeof(ElemT) * W.size());
 }
 
+// restart.h:414-657 at one rank: read the rank-0 file (file + "000000") written by
+// SaveWavefunction, find every saved alpha and beta string in the current basis (exact match of
+// the bit_length words), and copy the saved amplitude of each pair present in both; the rest
+// of W is zero.  The reference leaves W zero when the file is missing; here a W with no
+// overlap at all raises std::runtime_error, since a zero start vector cannot start Davidson.
+template <typename ElemT>
+void LoadWavefunction(const std::string& file, const std::vector<std::vector<size_t>>& adet,
+                      const std::vector<std::vector<size_t>>& bdet, const size_t adet_comm_size,
+                      const size_t bdet_comm_size, const MPI_Comm h_comm, const MPI_Comm b_comm,
+                      const MPI_Comm t_comm, std::vector<ElemT>& W) {
+  amd_detail::require_single(h_comm);
+  amd_detail::require_single(b_comm);
+  amd_detail::require_single(t_comm);
+  if (adet_comm_size != 1 || bdet_comm_size != 1) throw std::invalid_argument("sbd_amd: comm sizes must be 1");
+  W.assign(adet.size() * bdet.size(), ElemT(0));
+  std::ostringstream name;
+  name << file << std::setw(6) << std::setfill('0') << 0;
+  std::ifstream f(name.str(), std::ios::binary);
+  if (!f.is_open()) throw std::runtime_error("LoadWavefunction: cannot open " + name.str());
+  size_t na = 0, nb = 0, len = 0;
+  f.read(reinterpret_cast<char*>(&na), sizeof(size_t));
+  f.read(reinterpret_cast<char*>(&nb), sizeof(size_t));
+  f.read(reinterpret_cast<char*>(&len), sizeof(size_t));
+  if (!f || len == 0) throw std::runtime_error("LoadWavefunction: bad header in " + name.str());
+  std::vector<std::vector<size_t>> la(na, std::vector<size_t>(len)), lb(nb, std::vector<size_t>(len));
+  for (auto& a : la) f.read(reinterpret_cast<char*>(a.data()), sizeof(size_t) * len);
+  for (auto& b : lb) f.read(reinterpret_cast<char*>(b.data()), sizeof(size_t) * len);
+  std::vector<ElemT> lw(na * nb);
+  f.read(reinterpret_cast<char*>(lw.data()), sizeof(ElemT) * lw.size());
+  if (!f) throw std::runtime_error("LoadWavefunction: truncated file " + name.str());
+  std::map<std::vector<size_t>, size_t> ia_of, ib_of;
+  for (size_t i = 0; i < adet.size(); ++i) ia_of.emplace(adet[i], i);
+  for (size_t i = 0; i < bdet.size(); ++i) ib_of.emplace(bdet[i], i);
+  std::vector<long long> ma(na, -1), mb(nb, -1);
+  for (size_t i = 0; i < na; ++i) {
+    auto it = ia_of.find(la[i]);
+    if (it != ia_of.end()) ma[i] = static_cast<long long>(it->second);
+  }
+  for (size_t i = 0; i < nb; ++i) {
+    auto it = ib_of.find(lb[i]);
+    if (it != ib_of.end()) mb[i] = static_cast<long long>(it->second);
+  }
+  bool any = false;
+  for (size_t i = 0; i < na; ++i) {
+    if (ma[i] < 0) continue;
+    for (size_t j = 0; j < nb; ++j) {
+      if (mb[j] < 0) continue;
+      const ElemT v = lw[i * nb + j];
+      W[static_cast<size_t>(ma[i]) * bdet.size() + static_cast<size_t>(mb[j])] = v;
+      any = any || v != ElemT(0);
+    }
+  }
+  if (!any) throw std::runtime_error("LoadWavefunction: the saved wavefunction has no overlap with this basis");
+}
+
 // ------------------------------------------------------------------------------ driver
 
 namespace tpb {
@@ -594,9 +651,8 @@ inline void diag(const MPI_Comm& comm, const SBD& sbd_data, const FCIDump& fcidu
                  std::vector<std::vector<double>>& one_p_rdm, std::vector<std::vector<double>>& two_p_rdm) {
   int mpi_rank = 0;
   MPI_Comm_rank(comm, &mpi_rank);
-  if (sbd_data.method != 0)
-    throw std::invalid_argument("sbd_amd: method 1 (stored Hamiltonian) is not provided; use method 0");
-  if (!loadname.empty()) throw std::invalid_argument("sbd_amd: LoadWavefunction is not provided");
+  if (sbd_data.method != 0 && sbd_data.method != 1)
+    throw std::invalid_argument("sbd_amd: method must be 0 (on the fly) or 1 (stored Hamiltonian)");
   int L = 0, N = 0;
   double I0 = 0.0;
   oneInt<double> I1;
@@ -616,9 +672,15 @@ inline void diag(const MPI_Comm& comm, const SBD& sbd_data, const FCIDump& fcidu
     std::cout << " Elapsed time for helper construction " << std::chrono::duration<double>(t1 - t0).count()
               << " (sec) " << std::endl;
   std::vector<double> W;
-  BasisInitVector(W, adet, bdet, 1, 1, h_comm, b_comm, t_comm, sbd_data.init);
+  if (loadname.empty()) BasisInitVector(W, adet, bdet, 1, 1, h_comm, b_comm, t_comm, sbd_data.init);
+  else LoadWavefunction(loadname, adet, bdet, 1, 1, h_comm, b_comm, t_comm, W);
   std::vector<double> hii;
   makeQChamDiagTerms(adet, bdet, bl, L, helper, I0, I1, I2, hii, h_comm, b_comm, t_comm);
+  if (sbd_data.method == 1) {  // makeQCham (qcham.h:10-312): H stored once, mult reads it
+    int64_t nnz = 0;
+    amd_detail::check(sbdx_store_hamiltonian(engine_of(helper).ctx, 1, &nnz));
+    if (mpi_rank == 0) std::cout << " Stored Hamiltonian elements " << nnz << std::endl;
+  }
   Davidson(hii, W, adet, bdet, bl, static_cast<size_t>(L), 1, 1, helper, I0, I1, I2, h_comm, b_comm, t_comm,
            sbd_data.max_it, sbd_data.max_nb, sbd_data.eps, sbd_data.max_time);
   std::vector<double> C(W.size(), 0.0);

@@ -86,6 +86,14 @@ int sbdx_sigma_device(sbdx_ctx *ctx, int prec, int nvec, const void *dC, void *d
 int sbdx_sigma_block_device(sbdx_ctx *ctx, int prec, int k, const void *dC, void *dS,
                             void *stream);
 
+/* Stored Hamiltonian (the reference's method 1: makeQCham, chemistry/tpb/qcham.h:10-312, with
+ * its stored-H mult, mult.h:13-123): enable = 1 builds H as CSR in HBM (diagonal, same-spin
+ * singles and doubles, alpha-single x beta-single pairs; 12 bytes per element, refused with
+ * SBDX_ERR_DEVICE beyond 80% of free memory) and every later sigma build of this context, in
+ * every solver, is a sparse matrix-vector product with it; enable = 0 frees it.  *nnz
+ * (optional): stored elements.  Setting new integrals drops it. */
+int sbdx_store_hamiltonian(sbdx_ctx *ctx, int enable, int64_t *nnz);
+
 /* Davidson options (defaults by sbdx_davidson_defaults = the reference's SBD struct,
  * chemistry/tpb/sbdiag.h:12-30, with the iteration cap raised to converge). */
 typedef struct {

@@ -23,7 +23,7 @@ EXPORTS = [
     "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
     "sbdx_mlp_train_defaults", "sbdx_mlp_record", "sbdx_mlp_loss_grad", "sbdx_mlp_forward",
     "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune",
-    "sbdx_batch_create", "sbdx_batch_davidson", "sbdx_batch_destroy",
+    "sbdx_batch_create", "sbdx_batch_davidson", "sbdx_batch_destroy", "sbdx_store_hamiltonian",
 ]
 MLP_NPARAM = 12609
 MLP_SAMPLE = np.dtype([("alpha", "<u8"), ("beta", "<u8"), ("xh", "<f4"), ("xr", "<f4"),
@@ -113,6 +113,7 @@ def lib():
     L.sbdx_batch_davidson.argtypes = [vp, C.POINTER(DavidsonOpts), C.POINTER(vp),
                                       C.POINTER(DavidsonResult)]
     L.sbdx_batch_destroy.argtypes = [vp]
+    L.sbdx_store_hamiltonian.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.sbdx_batch_destroy.restype = None
     for name in EXPORTS:
         if name not in ("sbdx_ndet", "sbdx_davidson_defaults", "sbdx_destroy", "sbdx_last_error",
@@ -294,6 +295,12 @@ class Engine:
         p = np.ascontiguousarray(params, dtype=np.float32)
         assert p.size == MLP_NPARAM
         check(lib().sbdx_mlp_set(self._h, p.ctypes.data))
+
+    def store_hamiltonian(self, enable=True):
+        """Build (or drop) the stored CSR Hamiltonian; returns the stored element count."""
+        nnz = C.c_int64()
+        check(lib().sbdx_store_hamiltonian(self._h, int(bool(enable)), C.byref(nnz)))
+        return nnz.value
 
     def autotune(self, prec, k, reps=2):
         """Time the mixed-pass variants for (prec, k) and keep the fastest; returns

@@ -438,6 +438,19 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
   });
 }
 
+int sbdx_store_hamiltonian(sbdx_ctx* c, int enable, int64_t* nnz) {
+  return guarded([&] {
+    need_ints(c);
+    if (enable) {
+      const int64_t m = sbdx::store_hamiltonian(*c);
+      if (nnz) *nnz = m;
+    } else {
+      sbdx::drop_stored(*c);
+      if (nnz) *nnz = 0;
+    }
+  });
+}
+
 int sbdx_batch_create(int count, sbdx_ctx* const* ctxs, sbdx_batch** out) {
   return guarded([&] {
     need(out != nullptr && ctxs != nullptr && count >= 1, "bad argument");

@@ -133,6 +133,12 @@ struct sbdx_ctx {
   sbdx::Workspace ws;
   sbdx::Prof prof;
   int mix_ept_tuned[2][4] = {};    // autotuned mixed-pass entries per thread [FP64|FP32][K slot]
+  bool stored = false;             // H stored as CSR (stored.hip): sigma builds use it
+  sbdx::DevBuf<int64_t> h_rowptr;
+  sbdx::DevBuf<uint32_t> h_col;
+  sbdx::DevBuf<double> h_val;
+  sbdx::DevBuf<float> h_val32;
+  int64_t h_nnz = 0;
   bool have_mlp = false;           // learned preconditioner set (sbdx_mlp_set)
   sbdx::DevBuf<float> mlp_w;       // its parameters, rounded to bfloat16
   sbdx::DevBuf<float> mlp_pa, mlp_pb;  // per-string first-layer sums (alpha includes b1)
@@ -224,6 +230,10 @@ void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipS
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
 void apply_sign(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 void apply_sign_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
+// stored Hamiltonian (stored.hip)
+int64_t store_hamiltonian(sbdx_ctx& c);
+void drop_stored(sbdx_ctx& c);
+void sigma_stored(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t s);
 // batched lockstep Davidson of many small problems (batch.hip)
 void batch_davidson(sbdx_batch& b, const sbdx_davidson_opts& o, double* const* W, sbdx_davidson_result* res);
 // learned preconditioner (mlp.hip): T[det*k+v] = R/(theta_v - H_ii) (1 + m), P = float|double

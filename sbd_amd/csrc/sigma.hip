@@ -566,6 +566,10 @@ std::vector<int> mix_ept_candidates(int K) {
 
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {
   if (!c.have_ints) throw Error(-2, "integrals not set");
+  if (c.stored) {
+    sigma_stored(c, prec, k, x, y, st);
+    return;
+  }
   if (prec == 0) sigma_dispatch<double>(c, k, static_cast<const double*>(x), static_cast<double*>(y), st);
   else sigma_dispatch<float>(c, k, static_cast<const float*>(x), static_cast<float*>(y), st);
 }

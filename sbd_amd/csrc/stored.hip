@@ -1,0 +1,238 @@
+// stored.hip -- the stored-Hamiltonian path (the reference's method 1: makeQCham,
+// chemistry/tpb/qcham.h:10-312, and its mult, mult.h:13-123).
+//
+// The whole H (blocked basis) is built once as CSR on the GPU from the excitation lists: row
+// r = ia * nb + ib holds, in this order, the diagonal, the alpha doubles and singles, the beta
+// doubles and singles, and the alpha-single x beta-single pairs.  Row lengths come from the list
+// offsets, the row pointer from a device prefix sum, and one wavefront fills a row (its lanes
+// write consecutive entries).  Element values follow the on-the-fly kernels exactly (same
+// tables, same per-column Coulomb term).  A context with a stored H uses it for every sigma
+// build (any solver, any block width): one wavefront per row reads the row's (column, value)
+// pairs coalesced and gathers x.
+#include "common.h"
+#include "sbdx.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace sbdx {
+namespace {
+
+constexpr int kWaves = 4;  // wavefronts per workgroup
+
+struct Lists {
+  const int64_t* off_s;
+  const uint32_t* ket_s;
+  const uint32_t* exc_s;
+  const double* val_s;
+  const int64_t* off_d;
+  const uint32_t* ket_d;
+  const double* val_d;
+  const uint64_t* str;
+};
+
+__global__ void k_csr_len(Lists A, Lists B, int64_t na, int64_t nb, int64_t* __restrict__ len) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t n = na * nb;
+  if (r > n) return;
+  if (r == n) {
+    len[n] = 0;
+    return;
+  }
+  const int64_t ia = r / nb, ib = r - ia * nb;
+  const int64_t sa = A.off_s[ia + 1] - A.off_s[ia], da = A.off_d[ia + 1] - A.off_d[ia];
+  const int64_t sb = B.off_s[ib + 1] - B.off_s[ib], db = B.off_d[ib + 1] - B.off_d[ib];
+  len[r] = 1 + sa + da + sb + db + sa * sb;
+}
+
+// value of a single excitation entry q of one spin, with the spectator string's Coulomb term
+__device__ inline double single_value(const Lists& T, int64_t q, uint64_t spectator, const double* V, int L) {
+  const uint32_t e = T.exc_s[q];
+  const int64_t L2 = static_cast<int64_t>(L) * L;
+  const double* vp = V + (static_cast<int64_t>((e >> 6) & 63) * L + (e & 63)) * L2;
+  double coul = 0.0;
+  uint64_t o = spectator;
+  while (o) {
+    const int j = ctz64(o);
+    o &= o - 1;
+    coul += vp[j * L + j];
+  }
+  return T.val_s[q] + ((e >> 12) & 1u ? -coul : coul);
+}
+
+__global__ __launch_bounds__(64 * kWaves) void k_csr_fill(Lists A, Lists B, int64_t na, int64_t nb, int L,
+                                                          const double* __restrict__ V,
+                                                          const double* __restrict__ hii,
+                                                          const int64_t* __restrict__ rowptr,
+                                                          uint32_t* __restrict__ col, double* __restrict__ val) {
+  const int lane = threadIdx.x & 63;
+  const int64_t n = na * nb;
+  const int64_t L2 = static_cast<int64_t>(L) * L;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6); r < n;
+       r += static_cast<int64_t>(gridDim.x) * kWaves) {
+    const int64_t ia = r / nb, ib = r - ia * nb;
+    int64_t p = rowptr[r];
+    if (lane == 0) {
+      col[p] = static_cast<uint32_t>(r);
+      val[p] = hii[r];
+    }
+    ++p;
+    // alpha doubles and singles: (ja, ib)
+    const int64_t ad0 = A.off_d[ia], ad1 = A.off_d[ia + 1];
+    for (int64_t q = ad0 + lane; q < ad1; q += 64) {
+      col[p + q - ad0] = static_cast<uint32_t>(static_cast<int64_t>(A.ket_d[q]) * nb + ib);
+      val[p + q - ad0] = A.val_d[q];
+    }
+    p += ad1 - ad0;
+    const int64_t as0 = A.off_s[ia], as1 = A.off_s[ia + 1];
+    const uint64_t bstr = B.str[ib], astr = A.str[ia];
+    for (int64_t q = as0 + lane; q < as1; q += 64) {
+      col[p + q - as0] = static_cast<uint32_t>(static_cast<int64_t>(A.ket_s[q]) * nb + ib);
+      val[p + q - as0] = single_value(A, q, bstr, V, L);
+    }
+    p += as1 - as0;
+    // beta doubles and singles: (ia, jb)
+    const int64_t bd0 = B.off_d[ib], bd1 = B.off_d[ib + 1];
+    for (int64_t q = bd0 + lane; q < bd1; q += 64) {
+      col[p + q - bd0] = static_cast<uint32_t>(ia * nb + B.ket_d[q]);
+      val[p + q - bd0] = B.val_d[q];
+    }
+    p += bd1 - bd0;
+    const int64_t bs0 = B.off_s[ib], bs1 = B.off_s[ib + 1];
+    for (int64_t q = bs0 + lane; q < bs1; q += 64) {
+      col[p + q - bs0] = static_cast<uint32_t>(ia * nb + B.ket_s[q]);
+      val[p + q - bs0] = single_value(B, q, astr, V, L);
+    }
+    p += bs1 - bs0;
+    // alpha single x beta single: (ja, jb), s_a s_b (a i | b j)
+    const int64_t ns = bs1 - bs0, npair = (as1 - as0) * ns;
+    for (int64_t t = lane; t < npair; t += 64) {
+      const int64_t qa = as0 + t / ns, qb = bs0 + t % ns;
+      const uint32_t ea = A.exc_s[qa], eb = B.exc_s[qb];
+      const double v = V[(static_cast<int64_t>((ea >> 6) & 63) * L + (ea & 63)) * L2 +
+                         static_cast<int64_t>((eb >> 6) & 63) * L + (eb & 63)];
+      const bool neg = (((ea >> 12) ^ (eb >> 12)) & 1u) != 0;
+      col[p + t] = static_cast<uint32_t>(static_cast<int64_t>(A.ket_s[qa]) * nb + B.ket_s[qb]);
+      val[p + t] = neg ? -v : v;
+    }
+  }
+}
+
+__global__ void k_to_float(const double* __restrict__ in, float* __restrict__ out, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[i] = static_cast<float>(in[i]);
+}
+
+// y = H x for K interleaved vectors; one wavefront per row
+template <typename T, int K>
+__global__ __launch_bounds__(64 * kWaves) void k_spmv(const int64_t* __restrict__ rowptr, const uint32_t* __restrict__ col,
+                                                      const T* __restrict__ val, const T* __restrict__ x,
+                                                      T* __restrict__ y, int64_t n) {
+  const int lane = threadIdx.x & 63;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * kWaves + (threadIdx.x >> 6); r < n;
+       r += static_cast<int64_t>(gridDim.x) * kWaves) {
+    T acc[K];
+#pragma unroll
+    for (int v = 0; v < K; ++v) acc[v] = T(0);
+    const int64_t p0 = rowptr[r], p1 = rowptr[r + 1];
+    for (int64_t p = p0 + lane; p < p1; p += 64) {
+      const T w = val[p];
+      const int64_t c = col[p];
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[v] = fma(w, x[c * K + v], acc[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < K; ++v) {
+      T s = acc[v];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+      if (lane == 0) y[r * K + v] = s;
+    }
+  }
+}
+
+Lists lists(const SpinTab& t) {
+  return Lists{t.off_s.get(), t.ket_s.get(), t.exc_s.get(), t.val_s.get(), t.off_d.get(), t.ket_d.get(), t.val_d.get(),
+               t.str.get()};
+}
+
+}  // namespace
+
+int64_t store_hamiltonian(sbdx_ctx& c) {
+  if (!c.have_ints) throw Error(SBDX_ERR_STATE, "integrals not set");
+  const int64_t n = c.ndet();
+  if (n >= (int64_t(1) << 31)) throw Error(SBDX_ERR_ARG, "stored Hamiltonian: too many determinants");
+  hipStream_t st = c.stream;
+  DevBuf<int64_t> rowptr(n + 1), len(n + 1);
+  const Lists A = lists(c.A), B = lists(c.B);
+  k_csr_len<<<static_cast<unsigned>(cdiv(n + 1, 256)), 256, 0, st>>>(A, B, c.na, c.nb, len.get());
+  SBDX_CHECK_LAUNCH();
+  size_t tmp_bytes = 0;
+  SBDX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, len.get(), rowptr.get(), static_cast<int>(n + 1), st));
+  DevBuf<unsigned char> tmp(tmp_bytes);
+  SBDX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tmp_bytes, len.get(), rowptr.get(), static_cast<int>(n + 1), st));
+  int64_t nnz = 0;
+  SBDX_HIP(hipMemcpyAsync(&nnz, rowptr.get() + n, 8, hipMemcpyDeviceToHost, st));
+  SBDX_HIP(hipStreamSynchronize(st));
+  size_t free_b = 0, total_b = 0;
+  SBDX_HIP(hipMemGetInfo(&free_b, &total_b));
+  const double need = static_cast<double>(nnz) * 16.0;
+  if (need > 0.8 * static_cast<double>(free_b))
+    throw Error(SBDX_ERR_DEVICE, "stored Hamiltonian needs " + std::to_string(need / 1e9) + " GB (" +
+                                     std::to_string(nnz) + " elements); use the on-the-fly sigma");
+  c.h_col.alloc(nnz);
+  c.h_val.alloc(nnz);
+  c.h_val32.release();
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>(cdiv(n, kWaves), 65536));
+  k_csr_fill<<<g, 64 * kWaves, 0, st>>>(A, B, c.na, c.nb, c.L, c.eri4.get(), c.hii.get(), rowptr.get(), c.h_col.get(),
+                                        c.h_val.get());
+  SBDX_CHECK_LAUNCH();
+  SBDX_HIP(hipStreamSynchronize(st));
+  c.h_rowptr = std::move(rowptr);
+  c.h_nnz = nnz;
+  c.stored = true;
+  return nnz;
+}
+
+void drop_stored(sbdx_ctx& c) {
+  c.stored = false;
+  c.h_rowptr.release();
+  c.h_col.release();
+  c.h_val.release();
+  c.h_val32.release();
+  c.h_nnz = 0;
+}
+
+void sigma_stored(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {
+  const int64_t n = c.ndet();
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>(cdiv(n, kWaves), 65536));
+  auto go = [&](auto tag) {
+    using T = decltype(tag);
+    const T* val;
+    if constexpr (std::is_same<T, float>::value) {
+      if (!c.h_val32.get()) {
+        c.h_val32.alloc(c.h_nnz);
+        k_to_float<<<4096, 256, 0, st>>>(c.h_val.get(), c.h_val32.get(), c.h_nnz);
+        SBDX_CHECK_LAUNCH();
+      }
+      val = c.h_val32.get();
+    } else {
+      val = c.h_val.get();
+    }
+    const T* xx = static_cast<const T*>(x);
+    T* yy = static_cast<T*>(y);
+    switch (k) {
+      case 1: k_spmv<T, 1><<<g, 64 * kWaves, 0, st>>>(c.h_rowptr.get(), c.h_col.get(), val, xx, yy, n); break;
+      case 2: k_spmv<T, 2><<<g, 64 * kWaves, 0, st>>>(c.h_rowptr.get(), c.h_col.get(), val, xx, yy, n); break;
+      case 4: k_spmv<T, 4><<<g, 64 * kWaves, 0, st>>>(c.h_rowptr.get(), c.h_col.get(), val, xx, yy, n); break;
+      case 8: k_spmv<T, 8><<<g, 64 * kWaves, 0, st>>>(c.h_rowptr.get(), c.h_col.get(), val, xx, yy, n); break;
+      default: throw Error(SBDX_ERR_ARG, "block width must be 1, 2, 4 or 8");
+    }
+    SBDX_CHECK_LAUNCH();
+  };
+  if (prec == SBDX_FP32) go(float{});
+  else go(double{});
+}
+
+}  // namespace sbdx

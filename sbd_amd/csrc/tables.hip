@@ -237,6 +237,7 @@ void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta) {
 }
 
 void build_integral_tables(sbdx_ctx& c) {
+  drop_stored(c);  // a stored H belongs to the previous integrals
   const int L = c.L;
   const int n2 = 2 * L;
   const int64_t L2 = static_cast<int64_t>(L) * L, L4 = L2 * L2;

@@ -68,7 +68,11 @@ int main(int argc, char** argv) {
   std::vector<double> density;
   std::vector<std::vector<size_t>> co;
   std::vector<std::vector<double>> one, two;
-  sbd::tpb::diag(MPI_COMM_WORLD, d, fcidump, adet, bdet, "", out + ".wf", energy, density, co, one, two);
+  // --load <file>: start from a saved wavefunction (LoadWavefunction) instead of HF
+  std::string load;
+  for (int i = 4; i + 1 < argc; ++i)
+    if (std::string(argv[i]) == "--load") load = argv[i + 1];
+  sbd::tpb::diag(MPI_COMM_WORLD, d, fcidump, adet, bdet, load, out + ".wf", energy, density, co, one, two);
 
   // a second use of the API pieces: mult on a fixed vector, accumulated onto ones
   int LL, N;

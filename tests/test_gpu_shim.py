@@ -64,3 +64,49 @@ def test_cpp_drop_in_with_the_block_mixed_solver(tmp_path):
     ref = O.System.from_instance(inst).davidson(max_it=200, num_block=10, eps=1e-8)
     assert abs(res["energy"] - ref["energy"]) < 1e-9
     assert "(mixed)" in r.stdout
+
+
+def test_cpp_restart_from_saved_wavefunction(tmp_path):
+    """SaveWavefunction then LoadWavefunction (restart.h:20-67, :414-657): a second diag started
+    from the first one's file converges at once to the same energy; a basis with extra strings
+    gets the saved amplitudes on the common determinants and zeros elsewhere."""
+    if not os.path.exists(BIN):
+        pytest.skip("tests/cpp/shim_diag not built (needs the image's MPI)")
+    inst = synth.make_custom(10, 3, 3, 8, 60, 50, seed_ints=3)
+    vals, idx = inst.ints.fcidump_entries()
+    path = str(tmp_path / "inst.bin")
+    O.write_instance(path, inst.norb, 3, 3, inst.alpha, inst.beta, vals, idx)
+    first = str(tmp_path / "first")
+    r1 = subprocess.run([BIN, path, first, "1e-8"], capture_output=True, text=True, timeout=240)
+    assert r1.returncode == 0, r1.stderr[-2000:]
+    e1 = json.loads(r1.stdout.strip().splitlines()[-1])["energy"]
+    second = str(tmp_path / "second")
+    r2 = subprocess.run([BIN, path, second, "1e-8", "--load", first + ".wf"], capture_output=True,
+                        text=True, timeout=240)
+    assert r2.returncode == 0, r2.stderr[-2000:]
+    e2 = json.loads(r2.stdout.strip().splitlines()[-1])["energy"]
+    assert abs(e1 - e2) < 1e-9
+    steps1 = r1.stdout.count("Davidson iteration")
+    steps2 = r2.stdout.count("Davidson iteration")
+    assert steps2 < steps1, (steps1, steps2)
+    saved = np.fromfile(first + ".wf000000", dtype=np.uint64)
+    na, nb, ln = (int(v) for v in saved[:3])
+    assert (na, nb, ln) == (60, 50, 1)
+
+
+def test_cpp_drop_in_stored_hamiltonian(tmp_path):
+    """--method 1 (the reference's stored-Hamiltonian method): the drop-in stores H as CSR on
+    the GPU and its Davidson reaches the same energy."""
+    if not os.path.exists(BIN):
+        pytest.skip("tests/cpp/shim_diag not built (needs the image's MPI)")
+    inst = synth.make_custom(10, 3, 3, 8, 60, 50, seed_ints=3)
+    vals, idx = inst.ints.fcidump_entries()
+    path = str(tmp_path / "inst.bin")
+    O.write_instance(path, inst.norb, 3, 3, inst.alpha, inst.beta, vals, idx)
+    out = str(tmp_path / "shim")
+    r = subprocess.run([BIN, path, out, "1e-8", "--method", "1"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    ref = O.System.from_instance(inst).davidson(max_it=200, num_block=10, eps=1e-8)
+    assert abs(res["energy"] - ref["energy"]) < 1e-9
+    assert "Stored Hamiltonian elements" in r.stdout
