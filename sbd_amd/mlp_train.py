@@ -1,7 +1,7 @@
 """Training pipeline of the learned Davidson preconditioner (GPU; the kernels are in libsbdx).
 
     python -m sbd_amd.mlp_train --family c3 --strings 2000 --train 100-115 --eval 200-203 \\
-        --out sbd_amd/weights/c3.npy
+        --eval-strings 10000 --out sbd_amd/weights/c3_family.npy
 
 For every training seed of a workload family (integrals, alpha and beta strings all drawn from
 that seed): solve to convergence (block Davidson, mixed precision), then rerun from the HF
@@ -107,6 +107,8 @@ def main(argv=None):
     ap.add_argument("--strings", type=int, default=2000)
     ap.add_argument("--train", default="100-115")
     ap.add_argument("--eval", default="200-203")
+    ap.add_argument("--eval-strings", type=int, default=None,
+                    help="strings per spin of the held-out instances (default: --strings)")
     ap.add_argument("--target", type=float, default=50000.0, help="samples per snapshot")
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=16384)
@@ -142,13 +144,13 @@ def main(argv=None):
                   collect_s=round(t_collect, 2), train_s=round(t_train, 2),
                   loss=[float(v) for v in trace], eval={})
     for sd in seeds(args.eval):
-        inst = family_instance(args.family, args.strings, sd)
+        inst = family_instance(args.family, args.eval_strings or args.strings, sd)
         eng = Engine.from_instance(inst)
         import torch
         x = torch.zeros(eng.ndet, dtype=torch.float64, device="cuda")
         ref = solve(eng, x.data_ptr(), eps=1e-9)
         ev = evaluate(eng, params, ref["energy"])
-        report["eval"][sd] = dict(energy=ref["energy"], diagonal=ev[0], learned=ev[1])
+        report["eval"][sd] = dict(ndet=inst.ndet, energy=ref["energy"], diagonal=ev[0], learned=ev[1])
         log(f"[eval seed {sd}] diagonal {ev[0]['steps_to_tol']} steps {ev[0]['seconds_to_tol']:.2f}s, "
             f"learned {ev[1]['steps_to_tol']} steps {ev[1]['seconds_to_tol']:.2f}s "
             f"(fallback at {ev[1]['fallback']})")
