@@ -9,7 +9,8 @@
 // one copy, solves the subspace eigenproblems, and writes the coefficients back in one copy.
 // Graphs are instantiated once per (block index, active set) and replayed: for a step the
 // launch count is one whatever the number of instances, and instances of very different sizes
-// share the GPU inside one launch.
+// share the GPU inside one launch: each problem's sigma build is its own branch of the graph
+// (captured on side streams), so small problems run beside large ones.
 //
 // Batched kernels take grid (workgroups per instance, instances); instance i grid-strides over
 // its own n_i, and an instance that has converged is skipped by its workgroups at once (a device
@@ -73,6 +74,16 @@ __device__ inline void block_store(double (&acc)[M], int m, double* part, int sl
   __syncthreads();
 }
 
+// the instance's block pointers, staged in LDS once per workgroup (the loops below index them
+// per element; from global memory they would be reloaded every iteration)
+__device__ inline void stage_ptrs(const Inst& I, int nb, const void** sB, const void** sHB) {
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    sB[k] = I.B[k];
+    sHB[k] = I.HB[k];
+  }
+  __syncthreads();
+}
+
 // res[slot0 + q] = sum over workgroups of part[g][slot0 + q], one thread per q per instance
 __global__ void k_breduce(const Inst* __restrict__ in, int G, int slot0, int m) {
   const Inst& I = in[blockIdx.y];
@@ -90,15 +101,20 @@ template <typename P>
 __global__ __launch_bounds__(kB) void k_bcombine(const Inst* __restrict__ in, int nb, int mode) {
   const Inst& I = in[blockIdx.y];
   if (!*I.active) return;
-  const double* u = I.coef;
+  __shared__ const void* sB[kMaxBlocks];
+  __shared__ const void* sHB[kMaxBlocks];
+  __shared__ double u[kMaxBlocks];
+  stage_ptrs(I, nb, sB, sHB);
+  for (int k = threadIdx.x; k < nb; k += kB) u[k] = I.coef[k];
+  __syncthreads();
   const double e = I.coef[kMaxBlocks];
   double acc[1] = {0.0};
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < I.n;
        i += static_cast<int64_t>(gridDim.x) * kB) {
     double y = u[0] * I.X[i], hy = mode == 0 ? u[0] * I.HX[i] : 0.0;
     for (int k = 1; k < nb; ++k) {
-      y = fma(u[k], static_cast<double>(static_cast<const P*>(I.B[k])[i]), y);
-      if (mode == 0) hy = fma(u[k], static_cast<double>(static_cast<const P*>(I.HB[k])[i]), hy);
+      y = fma(u[k], static_cast<double>(static_cast<const P*>(sB[k])[i]), y);
+      if (mode == 0) hy = fma(u[k], static_cast<double>(static_cast<const P*>(sHB[k])[i]), hy);
     }
     const double o = mode == 0 ? hy - e * y : y;
     I.R[i] = o;
@@ -126,14 +142,17 @@ template <typename P>
 __global__ __launch_bounds__(kB) void k_bdots_t(const Inst* __restrict__ in, int j) {
   const Inst& I = in[blockIdx.y];
   if (!*I.active) return;
-  const P* T = static_cast<const P*>(I.B[j]);
+  __shared__ const void* sB[kMaxBlocks];
+  __shared__ const void* sHB[kMaxBlocks];
+  stage_ptrs(I, j + 1, sB, sHB);
+  const P* T = static_cast<const P*>(sB[j]);
   double acc[kMaxBlocks + 1];
   for (int k = 0; k <= j; ++k) acc[k] = 0.0;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < I.n;
        i += static_cast<int64_t>(gridDim.x) * kB) {
     const double t = static_cast<double>(T[i]);
     acc[0] = fma(I.X[i], t, acc[0]);
-    for (int k = 1; k < j; ++k) acc[k] = fma(static_cast<double>(static_cast<const P*>(I.B[k])[i]), t, acc[k]);
+    for (int k = 1; k < j; ++k) acc[k] = fma(static_cast<double>(static_cast<const P*>(sB[k])[i]), t, acc[k]);
     acc[j] = fma(t, t, acc[j]);
   }
   block_store<kMaxBlocks + 1>(acc, j + 1, I.part, 0);
@@ -155,11 +174,14 @@ __global__ __launch_bounds__(kB) void k_bproject(const Inst* __restrict__ in, in
     scale = nn > 0.0 ? 1.0 / sqrt(nn) : 0.0;
   }
   __syncthreads();
-  P* T = static_cast<P*>(I.B[j]);
+  __shared__ const void* sB[kMaxBlocks];
+  __shared__ const void* sHB[kMaxBlocks];
+  stage_ptrs(I, j + 1, sB, sHB);
+  P* T = static_cast<P*>(const_cast<void*>(sB[j]));
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < I.n;
        i += static_cast<int64_t>(gridDim.x) * kB) {
     double t = static_cast<double>(T[i]) - c[0] * I.X[i];
-    for (int k = 1; k < j; ++k) t = fma(-c[k], static_cast<double>(static_cast<const P*>(I.B[k])[i]), t);
+    for (int k = 1; k < j; ++k) t = fma(-c[k], static_cast<double>(static_cast<const P*>(sB[k])[i]), t);
     T[i] = static_cast<P>(t * scale);
   }
 }
@@ -170,8 +192,11 @@ template <typename P>
 __global__ __launch_bounds__(kB) void k_brow(const Inst* __restrict__ in, int j) {
   const Inst& I = in[blockIdx.y];
   if (!*I.active) return;
-  const P* Bj = static_cast<const P*>(I.B[j]);
-  const P* Hj = static_cast<const P*>(I.HB[j]);
+  __shared__ const void* sB[kMaxBlocks];
+  __shared__ const void* sHB[kMaxBlocks];
+  stage_ptrs(I, j + 1, sB, sHB);
+  const P* Bj = static_cast<const P*>(sB[j]);
+  const P* Hj = static_cast<const P*>(sHB[j]);
   double h[kMaxBlocks], s[kMaxBlocks];
   for (int k = 0; k <= j; ++k) h[k] = s[k] = 0.0;
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < I.n;
@@ -180,7 +205,7 @@ __global__ __launch_bounds__(kB) void k_brow(const Inst* __restrict__ in, int j)
     h[0] = fma(I.HX[i], b, h[0]);
     s[0] = fma(I.X[i], b, s[0]);
     for (int k = 1; k <= j; ++k) {
-      const double bk = static_cast<double>(static_cast<const P*>(I.B[k])[i]);
+      const double bk = static_cast<double>(static_cast<const P*>(sB[k])[i]);
       h[k] = fma(bk, hb, h[k]);
       s[k] = fma(bk, b, s[k]);
     }
@@ -312,6 +337,7 @@ struct BatchSolver {
     h_res.assign(static_cast<size_t>(cnt) * kSlots, 0.0);
     Hs.assign(cnt, std::vector<double>(m * m, 0.0));
     Ss = Hs;
+    make_side_streams();
     // warm every instance's sigma path eagerly: tables, FP32 copies and workspaces are
     // allocated outside graph capture
     for (int i = 0; i < cnt; ++i) {
@@ -322,6 +348,39 @@ struct BatchSolver {
   }
   ~BatchSolver() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    for (auto e : join_ev) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    for (auto s : side) (void)hipStreamDestroy(s);
+  }
+
+  // The problems' sigma builds are independent: inside a capture they go to kSide side streams
+  // (fork from and join back to the main stream through events), so each problem's chain is its
+  // own branch of the graph and small problems run beside large ones.
+  static constexpr int kSide = 8;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join_ev;
+  hipEvent_t fork_ev = nullptr;
+  void make_side_streams() {
+    side.resize(kSide);
+    join_ev.resize(kSide);
+    for (int q = 0; q < kSide; ++q) {
+      SBDX_HIP(hipStreamCreate(&side[q]));  // default priority
+      SBDX_HIP(hipEventCreateWithFlags(&join_ev[q], hipEventDisableTiming));
+    }
+    SBDX_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+  }
+  // sigma of every active problem, in P or FP64, from x_of(i) into y_of(i)
+  template <class FX, class FY>
+  void sigma_all(int sprec, FX x_of, FY y_of) {
+    SBDX_HIP(hipEventRecord(fork_ev, st));
+    for (int q = 0; q < kSide; ++q) SBDX_HIP(hipStreamWaitEvent(side[q], fork_ev, 0));
+    int slot = 0;
+    for (int i = 0; i < cnt; ++i)
+      if (active[i]) sigma_block(*b.ctx[i], sprec, 1, x_of(i), y_of(i), side[slot++ % kSide]);
+    for (int q = 0; q < kSide; ++q) {
+      SBDX_HIP(hipEventRecord(join_ev[q], side[q]));
+      SBDX_HIP(hipStreamWaitEvent(st, join_ev[q], 0));
+    }
   }
 
   dim3 grid() const { return dim3(G, static_cast<unsigned>(cnt)); }
@@ -368,8 +427,7 @@ struct BatchSolver {
   // anchor phase: sigma of every active X in FP64 and its Rayleigh data
   void anchor_phase() {
     graph(1000, [&] {
-      for (int i = 0; i < cnt; ++i)
-        if (active[i]) sigma_block(*b.ctx[i], SBDX_FP64, 1, X[i].get(), HX[i].get(), st);
+      sigma_all(SBDX_FP64, [&](int i) { return X[i].get(); }, [&](int i) { return HX[i].get(); });
       k_banchor<<<grid(), kB, 0, st>>>(dinst.get());
       SBDX_CHECK_LAUNCH();
       reduce(0, 1);
@@ -400,8 +458,7 @@ struct BatchSolver {
         k_bproject<P><<<grid(), kB, 0, st>>>(dinst.get(), j);
         SBDX_CHECK_LAUNCH();
       }
-      for (int i = 0; i < cnt; ++i)
-        if (active[i]) sigma_block(*b.ctx[i], prec, 1, B[i][j].get(), HB[i][j].get(), st);
+      sigma_all(prec, [&](int i) { return B[i][j].get(); }, [&](int i) { return HB[i][j].get(); });
       k_brow<P><<<grid(), kB, 0, st>>>(dinst.get(), j);
       SBDX_CHECK_LAUNCH();
       reduce(0, j + 1);
