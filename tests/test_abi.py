@@ -58,3 +58,39 @@ def test_null_arguments():
     lib = _lib.lib()
     assert lib.sbdx_create(4, 1, np.array([3], np.uint64), 1, np.array([3], np.uint64), None) == -1
     assert lib.sbdx_ndet(None) == 0
+
+
+STRUCTS = {
+    "sbdx_davidson_opts": _lib.DavidsonOpts,
+    "sbdx_davidson_result": _lib.DavidsonResult,
+    "sbdx_mlp_train_opts": _lib.MlpTrainOpts,
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof and every field offset of the ABI structs, as a C compiler lays them out from
+    include/sbdx.h, equal the Python binding's ctypes layouts (and the sample record's numpy
+    dtype)."""
+    import subprocess
+    src = ['#include <stddef.h>', '#include <stdio.h>', '#include "sbdx.h"', "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        src.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            src.append(f'  printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    src.append('  printf("sbdx_mlp_sample %zu\\n", sizeof(sbdx_mlp_sample));')
+    for f in _lib.MLP_SAMPLE.names:
+        src.append(f'  printf("sbdx_mlp_sample.{f} %zu\\n", offsetof(sbdx_mlp_sample, {f}));')
+    src.append("  return 0;\n}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                       check=True).stdout.splitlines())
+    for cname, cls in STRUCTS.items():
+        assert int(got[cname]) == C.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert int(got[f"{cname}.{fname}"]) == getattr(cls, fname).offset, (cname, fname)
+    assert int(got["sbdx_mlp_sample"]) == _lib.MLP_SAMPLE.itemsize
+    for f in _lib.MLP_SAMPLE.names:
+        assert int(got[f"sbdx_mlp_sample.{f}"]) == _lib.MLP_SAMPLE.fields[f][1], f

@@ -1,9 +1,12 @@
 """The seeded workload generator: deterministic, and the recipe it states."""
 import math
+import os
 
 import numpy as np
 
 from sbd_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_integrals_recipe():
@@ -52,3 +55,29 @@ def test_fcidump_entries_cover_store():
     B = 15
     assert two.sum() == B * (B + 1) // 2
     assert ((idx[:, 2] == 0) & (idx[:, 0] > 0)).sum() == 15
+
+
+def test_batch_instances_are_seeded_and_in_range():
+    a = synth.make_batch_instances(12, seed=4, dets=(1e3, 1e5))
+    b = synth.make_batch_instances(12, seed=4, dets=(1e3, 1e5))
+    for x, y in zip(a, b):
+        assert x.norb == y.norb and np.array_equal(x.alpha, y.alpha) and np.array_equal(x.beta, y.beta)
+        assert 10 <= x.norb <= 20 and x.n_alpha_el == x.norb // 3 == x.n_beta_el
+        assert x.meta["rank"] == 2 * x.norb
+        pc = [bin(int(s)).count("1") for s in x.alpha]
+        assert set(pc) == {x.n_alpha_el}
+        assert x.ndet >= 4
+    assert len({x.meta["seed_ints"] for x in a}) == 12
+
+
+def test_bench_solver_variants_are_valid_options():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    from sbd_amd._lib import DavidsonOpts
+    fields = {f for f, _ in DavidsonOpts._fields_}
+    for name, opts in bench.SOLVERS.items():
+        assert set(opts) - {"mlp"} <= fields, name
+        if "mlp" in opts:
+            assert os.path.exists(os.path.join(ROOT, opts["mlp"])), name
