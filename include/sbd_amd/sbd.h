@@ -12,7 +12,9 @@
 //   FCIDump, LoadFCIDump           framework/fcidump.h:18-139
 //   SetupIntegrals                 chemistry/basic/makeintegrals.h:10-55
 //   from_string, makestring,       framework/bit_manipulation.h:135-172, :324-330
-//   sort_bitarray, LoadAlphaDets   chemistry/basic/makedeterminants.h:160-185 (text files)
+//   sort_bitarray, LoadAlphaDets   chemistry/basic/makedeterminants.h:160-185 (.txt and .bin)
+//   DecodeAlphaDets                chemistry/basic/makedeterminants.h:63-108
+//   change_bitlength, get_extension framework/bit_manipulation.h:863-934, :1017-1038
 //   TaskHelpers, TaskCommunicator, chemistry/tpb/helper.h:12-36, :184-218, :432-574, :650-953
 //   MakeHelpers, RemakeHelpers, FreeHelpers
 //   makeQChamDiagTerms             chemistry/tpb/qcham.h:314-366
@@ -27,6 +29,7 @@
 //   tpb::SBD, generate_sbd_data,   chemistry/tpb/sbdiag.h:12-85, :102-456
 //   tpb::diag
 //   LoadWavefunction               chemistry/tpb/restart.h:414-657 (one rank's file)
+//   SaveMatrixFormWF               chemistry/tpb/restart.h:662-780 (.bin, .dat, .txt)
 //   tpb::diag method 1             makeQCham + stored-H mult (qcham.h:10-312, mult.h:13-123)
 //                                  as sbdx_store_hamiltonian
 // Not provided here: the generic operator layer (include/sbd/basic/).
@@ -36,6 +39,7 @@
 #include <mpi.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -209,15 +213,63 @@ inline void sort_bitarray(std::vector<std::vector<size_t>>& a) {
   a.erase(std::unique(a.begin(), a.end()), a.end());
 }
 
+// lower-cased text after the last '.', empty without one (bit_manipulation.h:1017-1038)
+inline std::string get_extension(const std::string& path) {
+  const size_t dot = path.rfind('.');
+  if (dot == std::string::npos) return std::string();
+  std::string ext = path.substr(dot + 1);
+  for (auto& ch : ext) ch = static_cast<char>(std::tolower(static_cast<unsigned char>(ch)));
+  return ext;
+}
+
+// Repack a bit string held bit_length_a bits per word into bit_length_b bits per word; bit q of
+// the string is bit q % w of word q / w, and the result has ceil(size * a / b) words
+// (bit_manipulation.h:863-893, and the overload over a set of strings at :901-934).
+inline void change_bitlength(size_t bit_length_a, std::vector<size_t>& b, size_t bit_length_b) {
+  const size_t nbits = b.size() * bit_length_a;
+  std::vector<size_t> out((nbits + bit_length_b - 1) / bit_length_b, 0);
+  for (size_t q = 0; q < nbits; ++q)
+    if ((b[q / bit_length_a] >> (q % bit_length_a)) & 1) out[q / bit_length_b] |= size_t(1) << (q % bit_length_b);
+  b.swap(out);
+}
+inline void change_bitlength(size_t bit_length_a, std::vector<std::vector<size_t>>& b, size_t bit_length_b) {
+  for (auto& v : b) change_bitlength(bit_length_a, v, bit_length_b);
+}
+
+// Binary determinant file: back-to-back records of ceil(norb / 8) bytes, each the string as a
+// big-endian unsigned integer whose bit p is orbital p (the top 8 ceil(norb / 8) - norb bits of
+// the first byte are padding and dropped); a trailing partial record is ignored.  Each string
+// comes back one bit per word, entry p = orbital p (makedeterminants.h:63-108).
+inline std::vector<std::vector<size_t>> DecodeAlphaDets(const std::string& filename, size_t norb) {
+  std::ifstream in(filename, std::ios::binary);
+  if (!in) throw std::runtime_error("Failed to open file");
+  const size_t nbyte = (norb + 7) / 8;
+  std::vector<std::vector<size_t>> all;
+  std::vector<unsigned char> rec(nbyte);
+  while (in.read(reinterpret_cast<char*>(rec.data()), static_cast<std::streamsize>(nbyte))) {
+    std::vector<size_t> bits(norb);
+    for (size_t p = 0; p < norb; ++p) bits[p] = (rec[nbyte - 1 - p / 8] >> (p % 8)) & 1;
+    all.push_back(std::move(bits));
+  }
+  return all;
+}
+
+// .txt: one string per line (from_string); .bin: DecodeAlphaDets then bit_length words.  Any
+// other extension loads nothing and leaves adet as given, as the reference does; either way
+// the result is sorted and de-duplicated (makedeterminants.h:160-185).
 inline void LoadAlphaDets(const std::string& file, std::vector<std::vector<size_t>>& adet,
                           size_t bit_length, size_t total_bit_length) {
-  if (file.size() < 4 || file.substr(file.size() - 4) != ".txt")
-    throw std::invalid_argument("LoadAlphaDets: only .txt determinant files are read here");
-  std::ifstream in(file);
-  if (!in.is_open()) throw std::runtime_error("Failed to open alpha det file.");
-  std::string line;
-  adet.clear();
-  while (std::getline(in, line)) adet.push_back(from_string(line, bit_length, total_bit_length));
+  const std::string ext = get_extension(file);
+  if (ext == "txt") {
+    std::ifstream in(file);
+    if (!in.is_open()) throw std::runtime_error("Failed to open alpha det file.");
+    std::string line;
+    adet.clear();
+    while (std::getline(in, line)) adet.push_back(from_string(line, bit_length, total_bit_length));
+  } else if (ext == "bin") {
+    adet = DecodeAlphaDets(file, total_bit_length);
+    change_bitlength(1, adet, bit_length);
+  }
   sort_bitarray(adet);
 }
 
@@ -591,6 +643,35 @@ void LoadWavefunction(const std::string& file, const std::vector<std::vector<siz
     }
   }
   if (!any) throw std::runtime_error("LoadWavefunction: the saved wavefunction has no overlap with this basis");
+}
+
+// restart.h:662-780 at one rank: the wavefunction as an |adet| x |bdet| matrix, alpha index
+// major.  .bin: the raw ElemT values in that order; .dat / .txt: one line "value # ia ib" per
+// entry, written with the stream's default 6 significant digits (the reference raises the
+// precision of std::cout, not of the file stream); other extensions write nothing.  A size
+// mismatch is reported on std::cout after writing, as in the reference.
+template <typename ElemT>
+void SaveMatrixFormWF(const std::string filename, std::vector<std::vector<size_t>>& adet,
+                      std::vector<std::vector<size_t>>& bdet, size_t adet_comm_size, size_t bdet_comm_size,
+                      MPI_Comm h_comm, MPI_Comm b_comm, MPI_Comm t_comm, std::vector<ElemT>& W) {
+  amd_detail::require_single(h_comm);
+  amd_detail::require_single(b_comm);
+  amd_detail::require_single(t_comm);
+  if (adet_comm_size != 1 || bdet_comm_size != 1) throw std::invalid_argument("sbd_amd: comm sizes must be 1");
+  std::cout.precision(16);
+  const std::string ext = get_extension(filename);
+  const size_t na = adet.size(), nb = bdet.size();
+  const size_t nw = std::min(W.size(), na * nb);
+  if (ext == "bin") {
+    std::ofstream f(filename, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(W.data()), static_cast<std::streamsize>(sizeof(ElemT) * nw));
+  } else if (ext == "dat" || ext == "txt") {
+    std::ofstream f(filename);
+    for (size_t q = 0; q < nw; ++q) f << W[q] << " # " << q / nb << " " << q % nb << std::endl;
+  } else {
+    return;
+  }
+  if (W.size() != na * nb) std::cout << " SaveMatrixFormWF: total size is not compatible " << std::endl;
 }
 
 // ------------------------------------------------------------------------------ driver

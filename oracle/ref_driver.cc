@@ -102,7 +102,8 @@ int main(int argc, char** argv) {
   MPI_Init_thread(&argc, &argv, MPI_THREAD_FUNNELED, &provided);
   MPI_Comm comm = MPI_COMM_WORLD;
 
-  std::string inpath, outp = "ref", op = "all", vecpath;
+  std::string inpath, outp = "ref", op = "all", vecpath, apath, bpath;
+  size_t norb = 0;
   int max_it = 100, max_nb = 10, reps = 1;
   double eps = 1e-8;
   size_t bit_length = 20;
@@ -118,6 +119,31 @@ int main(int argc, char** argv) {
     else if (a == "--tolerance") eps = std::stod(next());
     else if (a == "--bit_length") bit_length = std::stoul(next());
     else if (a == "--reps") reps = std::stoi(next());
+    else if (a == "--adets") apath = next();
+    else if (a == "--bdets") bpath = next();
+    else if (a == "--norb") norb = std::stoul(next());
+  }
+  if (op == "files") {
+    // determinant files through LoadAlphaDets (words written flat), and with --vec the
+    // wavefunction through SaveMatrixFormWF as .bin and .txt
+    std::vector<std::vector<size_t>> adet, bdet;
+    sbd::LoadAlphaDets(apath, adet, bit_length, norb);
+    sbd::LoadAlphaDets(bpath, bdet, bit_length, norb);
+    std::vector<size_t> fa, fb;
+    for (auto& v : adet) fa.insert(fa.end(), v.begin(), v.end());
+    for (auto& v : bdet) fb.insert(fb.end(), v.begin(), v.end());
+    write_vec(outp + ".adet.u64", fa);
+    write_vec(outp + ".bdet.u64", fb);
+    if (!vecpath.empty()) {
+      std::vector<double> W = read_vec(vecpath, adet.size() * bdet.size());
+      MPI_Comm h_comm, b_comm, t_comm;
+      sbd::TaskCommunicator(comm, 1, 1, 1, 1, h_comm, b_comm, t_comm);
+      sbd::SaveMatrixFormWF(outp + ".wf.bin", adet, bdet, 1, 1, h_comm, b_comm, t_comm, W);
+      sbd::SaveMatrixFormWF(outp + ".wf.txt", adet, bdet, 1, 1, h_comm, b_comm, t_comm, W);
+    }
+    std::printf("{\"op\":\"files\",\"na\":%zu,\"nb\":%zu}\n", adet.size(), bdet.size());
+    MPI_Finalize();
+    return 0;
   }
   Inst inst = read_instance(inpath);
 

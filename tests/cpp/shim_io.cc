@@ -1,0 +1,45 @@
+// Determinant and wavefunction files through the drop-in C++ API (include/sbd_amd/sbd.h):
+// sbd::LoadAlphaDets on a .txt or .bin file (words written flat, bit_length bits each) and,
+// with a vector file, sbd::SaveMatrixFormWF as .bin and .txt.  Host code only: no GPU call.
+// Test program: tests/test_shim_io.py compares its files with the reference's own.
+//
+//   shim_io <adets file> <bdets file> <norb> <bit_length> <out prefix> [vector file]
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "sbd_amd/sbd.h"
+
+int main(int argc, char** argv) {
+  int provided;
+  MPI_Init_thread(&argc, &argv, MPI_THREAD_FUNNELED, &provided);
+  if (argc < 6) {
+    std::fprintf(stderr, "usage: shim_io <adets> <bdets> <norb> <bit_length> <out prefix> [vector]\n");
+    return 2;
+  }
+  const size_t norb = std::stoul(argv[3]), bit_length = std::stoul(argv[4]);
+  const std::string out = argv[5];
+  std::vector<std::vector<size_t>> adet, bdet;
+  sbd::LoadAlphaDets(argv[1], adet, bit_length, norb);
+  sbd::LoadAlphaDets(argv[2], bdet, bit_length, norb);
+  auto dump = [](const std::string& path, const std::vector<std::vector<size_t>>& d) {
+    std::ofstream f(path, std::ios::binary);
+    for (const auto& v : d) f.write(reinterpret_cast<const char*>(v.data()), sizeof(size_t) * v.size());
+  };
+  dump(out + ".adet.u64", adet);
+  dump(out + ".bdet.u64", bdet);
+  if (argc > 6) {
+    std::vector<double> W(adet.size() * bdet.size());
+    std::ifstream f(argv[6], std::ios::binary);
+    f.read(reinterpret_cast<char*>(W.data()), static_cast<std::streamsize>(8 * W.size()));
+    if (!f) return 3;
+    MPI_Comm h_comm, b_comm, t_comm;
+    sbd::TaskCommunicator(MPI_COMM_WORLD, 1, 1, 1, 1, h_comm, b_comm, t_comm);
+    sbd::SaveMatrixFormWF(out + ".wf.bin", adet, bdet, 1, 1, h_comm, b_comm, t_comm, W);
+    sbd::SaveMatrixFormWF(out + ".wf.txt", adet, bdet, 1, 1, h_comm, b_comm, t_comm, W);
+  }
+  std::printf("{\"op\":\"files\",\"na\":%zu,\"nb\":%zu}\n", adet.size(), bdet.size());
+  MPI_Finalize();
+  return 0;
+}
