@@ -86,6 +86,73 @@ __global__ __launch_bounds__(kB) void k_gram(const TA* __restrict__ A, const TB*
   }
 }
 
+// Several blocks against one: partial[g][(q K + a) K + c] = sum over this workgroup's
+// determinants of L_q[det K + a] R[det K + c], for NL left blocks (FP64 or FP32 each) and one
+// right block.  Consecutive lanes take consecutive determinants, so every block is read in
+// whole lines and the right block once for all NL left ones.  The products are HBM-bound
+// (NL K^2 flops per NL K + K values read), so they run on the vector ALUs.
+struct LeftSet {
+  const void* p[32];
+  int dbl[32];  // 1: FP64 block, 0: FP32 block
+};
+
+template <typename TR, int K, int NL>
+__global__ __launch_bounds__(kB) void k_multigram(LeftSet L, const TR* __restrict__ R, int64_t n,
+                                                  double* __restrict__ partial) {
+  __shared__ double sh[kB / 64][NL * K * K];
+  double acc[NL][K][K];
+#pragma unroll
+  for (int q = 0; q < NL; ++q)
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int c = 0; c < K; ++c) acc[q][a][c] = 0.0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    double r[K];
+#pragma unroll
+    for (int c = 0; c < K; ++c) r[c] = static_cast<double>(R[i * K + c]);
+#pragma unroll
+    for (int q = 0; q < NL; ++q) {
+      double l[K];
+      if (L.dbl[q]) {
+#pragma unroll
+        for (int a = 0; a < K; ++a) l[a] = static_cast<const double*>(L.p[q])[i * K + a];
+      } else {
+#pragma unroll
+        for (int a = 0; a < K; ++a) l[a] = static_cast<double>(static_cast<const float*>(L.p[q])[i * K + a]);
+      }
+#pragma unroll
+      for (int a = 0; a < K; ++a)
+#pragma unroll
+        for (int c = 0; c < K; ++c) acc[q][a][c] = __fma_rn(l[a], r[c], acc[q][a][c]);
+    }
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < NL; ++q)
+#pragma unroll
+    for (int a = 0; a < K; ++a)
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const double s = wsum(acc[q][a][c]);
+        if (lane == 0) sh[w][(q * K + a) * K + c] = s;
+      }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NL * K * K; t += kB) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < kB / 64; ++ww) s += sh[ww][t];
+    partial[static_cast<int64_t>(blockIdx.x) * NL * K * K + t] = s;
+  }
+}
+
+// left blocks per k_multigram pass: NL K^2 accumulators per thread, at most 32 (64 for K = 8)
+template <int K>
+constexpr int multigram_max_nl() {
+  return 32 / (K * K) > 1 ? 32 / (K * K) : 1;
+}
+
 // out[q] = sum_b partial[b][q], one workgroup per q
 __global__ __launch_bounds__(kB) void k_sum_partials(const double* __restrict__ partial, int nblock, int m,
                                                     double* __restrict__ out) {
@@ -438,7 +505,7 @@ struct BlockSolver {
       B[i].alloc(n * K);
       HB[i].alloc(n * K);
     }
-    partial.alloc(static_cast<size_t>(2048) * K * K);
+    partial.alloc(static_cast<size_t>(2048) * std::max(K * K, 32));  // k_gram, k_multigram
     small.alloc(kMaxBlocks * K * K + K);
     Hs.assign(static_cast<size_t>(m) * K * m * K, 0.0);
     Ss = Hs;
@@ -475,6 +542,61 @@ struct BlockSolver {
     if (i == 0) gram(X.get(), b, out);
     else gram(static_cast<const P*>(bptr(i)), b, out);
   }
+  // out[i][a][c] (host, K x K per left block) = L_i^T R for every left block, in passes of up
+  // to multigram_max_nl<K>() left blocks (power-of-two groups) over one read of R each;
+  // lefts: (pointer, is FP64)
+  void multigram(const std::vector<std::pair<const void*, bool>>& lefts, const void* R, double* out) {
+    const unsigned g = nblk(n);
+    size_t b0 = 0;
+    while (b0 < lefts.size()) {
+      int nl = multigram_max_nl<K>();
+      while (static_cast<size_t>(nl) > lefts.size() - b0) nl /= 2;
+      LeftSet ls{};
+      for (int q = 0; q < nl; ++q) {
+        ls.p[q] = lefts[b0 + q].first;
+        ls.dbl[q] = lefts[b0 + q].second ? 1 : 0;
+      }
+      const P* r = static_cast<const P*>(R);
+      switch (nl) {
+        case 1: k_multigram<P, K, 1><<<g, kB, 0, st>>>(ls, r, n, partial.get()); break;
+        case 2: k_multigram<P, K, (multigram_max_nl<K>() >= 2 ? 2 : 1)><<<g, kB, 0, st>>>(ls, r, n, partial.get()); break;
+        case 4: k_multigram<P, K, (multigram_max_nl<K>() >= 4 ? 4 : 1)><<<g, kB, 0, st>>>(ls, r, n, partial.get()); break;
+        case 8: k_multigram<P, K, (multigram_max_nl<K>() >= 8 ? 8 : 1)><<<g, kB, 0, st>>>(ls, r, n, partial.get()); break;
+        case 16: k_multigram<P, K, (multigram_max_nl<K>() >= 16 ? 16 : 1)><<<g, kB, 0, st>>>(ls, r, n, partial.get()); break;
+        default: k_multigram<P, K, (multigram_max_nl<K>() >= 32 ? 32 : 1)><<<g, kB, 0, st>>>(ls, r, n, partial.get()); break;
+      }
+      SBDX_CHECK_LAUNCH();
+      k_sum_partials<<<nl * K * K, kB, 0, st>>>(partial.get(), static_cast<int>(g), nl * K * K, small.get());
+      SBDX_CHECK_LAUNCH();
+      SBDX_HIP(hipMemcpyAsync(out + b0 * K * K, small.get(), nl * K * K * 8, hipMemcpyDeviceToHost, st));
+      SBDX_HIP(hipStreamSynchronize(st));
+      b0 += nl;
+    }
+  }
+
+  // the new row j >= 1 of the subspace matrices in two passes: [X, HX, B_1..B_j]^T B_j (overlaps
+  // and the anchor's exact coupling (H X)^T B_j) and [B_1..B_j]^T HB_j
+  void fill_row(int j) {
+    std::vector<std::pair<const void*, bool>> ls{{X.get(), true}, {HX.get(), true}};
+    for (int i = 1; i <= j; ++i) ls.push_back({bptr(i), prec == SBDX_FP64});
+    std::vector<double> o1(ls.size() * K * K);
+    multigram(ls, bptr(j), o1.data());
+    std::vector<std::pair<const void*, bool>> lh;
+    for (int i = 1; i <= j; ++i) lh.push_back({bptr(i), prec == SBDX_FP64});
+    std::vector<double> o2(lh.size() * K * K);
+    multigram(lh, hbptr(j), o2.data());
+    const int ld = m * K;
+    auto put = [&](int i, const double* h, const double* sv) {
+      for (int a = 0; a < K; ++a)
+        for (int b = 0; b < K; ++b) {
+          Hs[(i * K + a) * ld + j * K + b] = Hs[(j * K + b) * ld + i * K + a] = h[a * K + b];
+          Ss[(i * K + a) * ld + j * K + b] = Ss[(j * K + b) * ld + i * K + a] = sv[a * K + b];
+        }
+    };
+    put(0, o1.data() + K * K, o1.data());
+    for (int i = 1; i <= j; ++i) put(i, o2.data() + (i - 1) * K * K, o1.data() + (i + 1) * K * K);
+  }
+
   // subspace matrices: blocks (i, j) and (j, i) from B_i^T HB_j and B_i^T B_j.  The anchor's
   // coupling to a correction block is taken as (H X)^T B_j with the exact FP64 sigma of X, not
   // as X^T (H B_j) with the correction's precision-P sigma: it is the first-order term that
@@ -610,7 +732,9 @@ struct BlockSolver {
     precondition(E, nr, nstep);
     for (int pass = 0; pass < 2; ++pass) {
       std::vector<double> coef(nblocks * K * K);
-      for (int i = 0; i < nblocks; ++i) gram_with(i, static_cast<const P*>(T.get()), coef.data() + i * K * K);
+      std::vector<std::pair<const void*, bool>> ls{{X.get(), true}};
+      for (int i = 1; i < nblocks; ++i) ls.push_back({bptr(i), prec == SBDX_FP64});
+      multigram(ls, T.get(), coef.data());
       upload(small.get(), coef.data(), coef.size());
       k_project_out<P, K><<<nblk(n), kB, 0, st>>>(T.get(), X.get(), ptrs(nblocks), nblocks, small.get(), n);
       SBDX_CHECK_LAUNCH();
@@ -641,8 +765,7 @@ struct BlockSolver {
     SBDX_CHECK_LAUNCH();
     std::swap(B[1], Pd);
     std::swap(HB[1], HPd);
-    fill_block(0, 1);
-    fill_block(1, 1);
+    fill_row(1);
     have_dir = false;
   }
 
@@ -744,7 +867,7 @@ struct BlockSolver {
         expand(nblocks, E, nr, nstep);
         std::swap(B[nblocks], T);
         sigma_block(c, prec, K, B[nblocks].get(), HB[nblocks].get(), st);
-        for (int i = 0; i <= nblocks; ++i) fill_block(i, nblocks);
+        fill_row(nblocks);
         ++nblocks;
       }
       // thick restart (or the final vectors): X <- normalised Ritz vectors of the subspace; the

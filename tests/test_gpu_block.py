@@ -55,7 +55,7 @@ def test_lowest_root_fp64(small, block):
     assert residuals(H, W, [evals[0]])[0] < 1e-8
 
 
-@pytest.mark.parametrize("nroots", [2, 4])
+@pytest.mark.parametrize("nroots", [2, 4, 8])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_several_roots(small, nroots, prec):
     name, so, eng, H, evals = small
