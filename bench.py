@@ -353,7 +353,7 @@ def main():
             tuned = None
             if args.autotune:
                 chosen, cand = eng.autotune(prec, k, reps=1)
-                tuned = {"entries_per_thread": chosen, "candidates_ms": cand}
+                tuned = {"variant": chosen, "candidates_ms": cand}
                 log(f"[bench] autotune block {k}: {tuned}")
             xb = torch.randn(N * k, dtype=dt, device="cuda", generator=g)
             yb = torch.empty_like(xb)

@@ -234,12 +234,19 @@ int sbdx_batch_davidson(sbdx_batch *batch, const sbdx_davidson_opts *opts, doubl
 void sbdx_batch_destroy(sbdx_batch *batch);
 
 /* Autotuner, no counterpart in the reference: times every variant of the sigma build's
- * alpha-single x beta-single kernel (entries per thread, which also sets its tile shape) for
- * block width k and precision prec on this context's problem, `reps` builds each, and keeps the
- * fastest for every later build of that (prec, k).  candidates/ms (optional, cap entries): the
- * variants and their mean ms per build; *chosen: the variant kept; *ncand: variants tried. */
+ * alpha-single x beta-single kernel (coded 100 * threads per workgroup + entries per thread;
+ * the two set its tile shape) for block width k and precision prec on this context's problem,
+ * `reps` builds each, and keeps the fastest for every later build of that (prec, k).
+ * candidates/ms (optional, cap entries): the variants and their mean ms per build; *chosen:
+ * the variant kept; *ncand: variants tried. */
 int sbdx_autotune(sbdx_ctx *ctx, int prec, int k, int reps, int *chosen, int *candidates,
                   double *ms, int cap, int *ncand);
+
+/* No counterpart in the reference: pin the alpha-single x beta-single kernel variant for
+ * (prec, k) to one of the autotuner's candidates (the values sbdx_autotune reports), or back to
+ * the built-in default with variant 0; SBDX_ERR_ARG for any other value.  For tests and for
+ * reproducible timing. */
+int sbdx_set_mix_variant(sbdx_ctx *ctx, int prec, int k, int variant);
 
 /* Measurement hook, no counterpart in the reference: when enabled, every sigma build records
  * HIP events on its launch stream around its phases (0 alpha rows, 1 alpha-single x

@@ -22,7 +22,7 @@ EXPORTS = [
     "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
     "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
     "sbdx_mlp_train_defaults", "sbdx_mlp_record", "sbdx_mlp_loss_grad", "sbdx_mlp_forward",
-    "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune",
+    "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune", "sbdx_set_mix_variant",
     "sbdx_batch_create", "sbdx_batch_davidson", "sbdx_batch_destroy", "sbdx_store_hamiltonian",
 ]
 MLP_NPARAM = 12609
@@ -109,6 +109,7 @@ def lib():
     i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
     L.sbdx_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), i32p, f64p,
                                 C.c_int, C.POINTER(C.c_int)]
+    L.sbdx_set_mix_variant.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.sbdx_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp)]
     L.sbdx_batch_davidson.argtypes = [vp, C.POINTER(DavidsonOpts), C.POINTER(vp),
                                       C.POINTER(DavidsonResult)]
@@ -304,12 +305,17 @@ class Engine:
 
     def autotune(self, prec, k, reps=2):
         """Time the mixed-pass variants for (prec, k) and keep the fastest; returns
-        (chosen entries per thread, {candidate: mean ms})."""
+        (chosen variant, {variant: mean ms}), variant = 100 * threads per workgroup + entries
+        per thread."""
         cand = np.zeros(8, np.int32)
         ms = np.zeros(8)
         chosen, nc = C.c_int(), C.c_int()
         check(lib().sbdx_autotune(self._h, prec, k, reps, C.byref(chosen), cand, ms, 8, C.byref(nc)))
         return chosen.value, {int(cand[i]): float(ms[i]) for i in range(nc.value)}
+
+    def set_mix_variant(self, prec, k, variant):
+        """Pin the mixed-pass variant for (prec, k) (one of autotune's candidates; 0: default)."""
+        check(lib().sbdx_set_mix_variant(self._h, prec, k, int(variant)))
 
     def mlp_apply(self, theta, r):
         t = np.zeros(self.ndet)

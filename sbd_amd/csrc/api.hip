@@ -6,6 +6,7 @@
 #include "common.h"
 #include "sbdx.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -402,8 +403,8 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
     hipEvent_t e0, e1;
     SBDX_HIP(hipEventCreate(&e0));
     SBDX_HIP(hipEventCreate(&e1));
-    const std::vector<int> cand = sbdx::mix_ept_candidates(k);
-    int& slot = c->mix_ept_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)];
+    const std::vector<int> cand = sbdx::mix_variant_candidates(k);
+    int& slot = c->mix_variant_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)];
     int best = 0;
     double best_ms = 0.0;
     try {
@@ -435,6 +436,18 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
     (void)hipEventDestroy(e1);
     if (chosen) *chosen = best;
     if (ncand) *ncand = static_cast<int>(cand.size());
+  });
+}
+
+int sbdx_set_mix_variant(sbdx_ctx* c, int prec, int k, int variant) {
+  return guarded([&] {
+    need(c != nullptr, "null context");
+    need((k == 1 || k == 2 || k == 4 || k == 8) && (prec == SBDX_FP64 || prec == SBDX_FP32),
+         "bad argument (k must be 1, 2, 4 or 8)");
+    const std::vector<int> cand = sbdx::mix_variant_candidates(k);
+    need(variant == 0 || std::find(cand.begin(), cand.end(), variant) != cand.end(),
+         "not a mixed-kernel variant for this block width");
+    c->mix_variant_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)] = variant;
   });
 }
 

@@ -100,7 +100,7 @@ struct Prof {
 struct MixTiles {
   int ntile = 0;
   int max_cols = 0;
-  int ept = 32;              // entries per thread of the mixed kernel (16 or 32)
+  int64_t cap = 0;           // entries per tile (threads x entries per thread); 0: empty
   DevBuf<int64_t> ent_off;   // ntile + 1
   DevBuf<int32_t> col0, ncol;
   DevBuf<uint32_t> jb, meta;
@@ -121,7 +121,7 @@ struct sbdx_ctx {
   int L = 0;
   int64_t na = 0, nb = 0;
   sbdx::SpinTab A, B;
-  sbdx::MixTiles mix[4];           // mixed-pass tiles for 4, 8, 16, 32 entries per thread
+  sbdx::MixTiles mix[6];           // mixed-pass tiles, one per tile size in use
   bool have_ints = false;
   double I0 = 0.0;
   std::vector<double> h_I1, h_I2;
@@ -132,7 +132,7 @@ struct sbdx_ctx {
   sbdx::DevBuf<float> hii32;
   sbdx::Workspace ws;
   sbdx::Prof prof;
-  int mix_ept_tuned[2][4] = {};    // autotuned mixed-pass entries per thread [FP64|FP32][K slot]
+  int mix_variant_tuned[2][4] = {}; // autotuned mixed-kernel variant [FP64|FP32][K slot]
   bool stored = false;             // H stored as CSR (stored.hip): sigma builds use it
   sbdx::DevBuf<int64_t> h_rowptr;
   sbdx::DevBuf<uint32_t> h_col;
@@ -223,7 +223,7 @@ void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta);  //
 void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
 void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
 void reset_mix_tiles(sbdx_ctx& c);              // sigma.hip: drop cached mixed-pass tiles
-std::vector<int> mix_ept_candidates(int k);     // sigma.hip: autotuner search space
+std::vector<int> mix_variant_candidates(int k); // sigma.hip: autotuner search space
 // sigma of nvec vectors stored one after another (vector-major)
 void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 // fused sigma of a block of k vectors stored interleaved, element [det * k + v], k in {1,2,4,8}

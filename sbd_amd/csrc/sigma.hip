@@ -86,7 +86,9 @@ struct RowTabs {
 };
 
 // Same-spin part (and, for the alpha rows, the diagonal): y[row, col] = sum over the row's
-// singles and doubles of value * x[ket, col], for K interleaved vectors.
+// singles and doubles of value * x[ket, col], for K interleaved vectors.  One workgroup = one
+// row x kTile columns: the row's list entries are the same for every lane (scalar loads), the
+// gathers of one entry are contiguous across the lanes.
 template <typename T, bool kDiag, int K>
 __global__ __launch_bounds__(kTile) void k_sigma_rows(
     int64_t nrow, int64_t ncol, int L, RowTabs rt, const T* __restrict__ val_s,
@@ -180,11 +182,16 @@ __global__ __launch_bounds__(kTile) void k_sigma_rows(
 // sorted jb.  Workgroups are numbered so that all tiles of one row run on the same XCD and
 // share that row's ket rows in L2.  Per-entry sums are folded into the tile's columns with LDS
 // atomics once at the end; sigma is updated by its owner, with no global atomics.
-constexpr int kMixThreads = 256;
 constexpr int kMixMaxCols = 1024;
-// V row elements fetched per thread: 8 covers L^2 <= 2048 (L <= 45), 16 covers L <= 64
+// Variants of the kernel: entries per thread (kEPT) and threads per workgroup (kMixThreads),
+// coded as 100 * threads + entries per thread.  Every workgroup of a row gathers from all of
+// the row's ket rows, touching most of their cache lines whatever its tile size, so larger
+// tiles (threads x entries) mean fewer line fetches per row; more entries per thread mean
+// more registers and fewer waves.  The autotuner picks among them per (precision, K).
+// V row elements fetched per thread: kVPerThread * kMixThreads covers L^2 (2048: L <= 45,
+// 4096: L <= 64)
 
-template <typename T, int kEPT, int kVPerThread, int K>
+template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads>
 __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     int64_t na, int64_t nb, int L, const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket,
     const uint32_t* __restrict__ a_exc, int ntile, const int64_t* __restrict__ t_off,
@@ -346,36 +353,32 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
   stk<T, K>(y + k, a);
 }
 
-// entries per thread of the mixed kernel for a block width K: the context's autotuned choice
-// when there is one (sbdx_autotune), else the default of the per-entry accumulator register
-// budget; for K = 1 the default may be overridden by SBDX_MIX_EPT
-int mix_ept(const sbdx_ctx& c, int prec, int K) {
-  const int tuned = c.mix_ept_tuned[prec == 0 ? 0 : 1][k_slot(K)];
+// variant of the mixed kernel (100 * threads + entries per thread) for a block width K: the
+// context's autotuned choice when there is one (sbdx_autotune), else the autotuner's choice on
+// the 4e8-determinant 36-orbital workload
+int mix_variant(const sbdx_ctx& c, int prec, int K) {
+  const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
   if (tuned) return tuned;
-  // defaults: the autotuner's choices on the 4e8-determinant 36-orbital workload
-  if (K == 1) {
-    const char* env = std::getenv("SBDX_MIX_EPT");
-    const int def = prec == 0 ? 16 : 32;
-    const int want = env ? std::atoi(env) : def;
-    return (want == 8 || want == 16 || want == 32) ? want : def;
-  }
-  if (K == 2) return prec == 0 ? 16 : 8;
-  return K == 4 ? 8 : 4;
+  if (K == 1) return prec == 0 ? 51216 : 25632;
+  if (K == 2) return prec == 0 ? 51216 : 102404;
+  if (K == 4) return prec == 0 ? 102404 : 25608;
+  return 25604;
 }
 
-int ept_slot(int ept) { return ept == 4 ? 0 : ept == 8 ? 1 : ept == 16 ? 2 : 3; }
-
-// Regroup the beta singles into mixed-pass tiles of at most 256 * ept entries (host side, once
-// per integral set and ept: the packed excitation words carry the orbital pair and sign).
-const MixTiles& mix_tiles(sbdx_ctx& c, int ept) {
-  MixTiles& mt = c.mix[ept_slot(ept)];
-  if (mt.ept == ept && mt.ent_off.get()) return mt;
+// Regroup the beta singles into mixed-pass tiles of at most cap entries (host side, once per
+// integral set and tile size: the packed excitation words carry the orbital pair and sign).
+const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
+  MixTiles* slot = nullptr;
+  for (auto& m : c.mix) {
+    if (m.cap == cap && m.ent_off.get()) return m;
+    if (!slot && m.cap == 0) slot = &m;
+  }
+  MixTiles& mt = slot ? *slot : c.mix[0];
   mt = MixTiles{};
-  mt.ept = ept;
+  mt.cap = cap;
   const SpinTab& B = c.B;
   const int L = c.L;
   const int64_t nb = B.n;
-  const int64_t cap = static_cast<int64_t>(kMixThreads) * ept;
   if (B.nnz_s == 0 || c.A.nnz_s == 0) {
     mt.ent_off.alloc(1);
     return mt;
@@ -434,39 +437,56 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int ept) {
 
 template <typename T, int K>
 void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
-  const int ept = mix_ept(c, std::is_same<T, double>::value ? 0 : 1, K);
-  const MixTiles& mt = mix_tiles(c, ept);
+  const int var = mix_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
+  const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * (var / 100));
   if (mt.ntile == 0) return;
   const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
   const size_t lds = (2 * static_cast<size_t>(c.L) * c.L + static_cast<size_t>(mt.max_cols) * K) * sizeof(T);
-  auto launch = [&](auto kernel) {
+  auto launch = [&](auto kernel, int threads) {
     if (lds > 65536)
       SBDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    kernel<<<static_cast<unsigned>(blocks), kMixThreads, lds, st>>>(
+    kernel<<<static_cast<unsigned>(blocks), threads, lds, st>>>(
         c.na, c.nb, c.L, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
         mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), V, x, y);
     SBDX_CHECK_LAUNCH();
   };
-  const bool small_v = static_cast<int>(c.L) * c.L <= 8 * kMixThreads;
-  auto pick = [&](auto e) {
-    constexpr int E = decltype(e)::value;
-    small_v ? launch(&k_sigma_mixed<T, E, 8, K>) : launch(&k_sigma_mixed<T, E, 16, K>);
+  const bool small_v = static_cast<int>(c.L) * c.L <= 2048;
+  auto pick = [&](auto e, auto t) {
+    constexpr int E = decltype(e)::value, NT = decltype(t)::value;
+    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT>, NT) : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT>, NT);
   };
   using std::integral_constant;
+  using I4 = integral_constant<int, 4>;
+  using I8 = integral_constant<int, 8>;
+  using I16 = integral_constant<int, 16>;
+  using I32 = integral_constant<int, 32>;
+  using T256 = integral_constant<int, 256>;
+  using T512 = integral_constant<int, 512>;
+  using T1024 = integral_constant<int, 1024>;
+  // the candidates of mix_variant_candidates(K); anything else is refused
   if constexpr (K == 1) {
-    if (ept == 8) pick(integral_constant<int, 8>{});
-    else if (ept == 16) pick(integral_constant<int, 16>{});
-    else pick(integral_constant<int, 32>{});
+    if (var == 25632) pick(I32{}, T256{});
+    else if (var == 25616) pick(I16{}, T256{});
+    else if (var == 51216) pick(I16{}, T512{});
+    else if (var == 102408) pick(I8{}, T1024{});
+    else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   } else if constexpr (K == 2) {
-    if (ept == 4) pick(integral_constant<int, 4>{});
-    else if (ept == 16) pick(integral_constant<int, 16>{});
-    else pick(integral_constant<int, 8>{});
+    if (var == 102404) pick(I4{}, T1024{});
+    else if (var == 25608) pick(I8{}, T256{});
+    else if (var == 102408) pick(I8{}, T1024{});
+    else if (var == 25616) pick(I16{}, T256{});
+    else if (var == 51216) pick(I16{}, T512{});
+    else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   } else if constexpr (K == 4) {
-    if (ept == 8) pick(integral_constant<int, 8>{});
-    else pick(integral_constant<int, 4>{});
+    if (var == 25608) pick(I8{}, T256{});
+    else if (var == 25604) pick(I4{}, T256{});
+    else if (var == 102404) pick(I4{}, T1024{});
+    else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   } else {
-    pick(integral_constant<int, 4>{});
+    if (var == 25604) pick(I4{}, T256{});
+    else if (var == 51204) pick(I4{}, T512{});
+    else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   }
 }
 
@@ -556,12 +576,13 @@ void reset_mix_tiles(sbdx_ctx& c) {
   for (auto& m : c.mix) m = MixTiles{};
 }
 
-// entries-per-thread candidates of the mixed kernel for block width K
-std::vector<int> mix_ept_candidates(int K) {
-  if (K == 1) return {8, 16, 32};
-  if (K == 2) return {4, 8, 16};
-  if (K == 4) return {4, 8};
-  return {4};
+// variants (100 * threads + entries per thread) of the mixed kernel for block width K: none
+// spills at its workgroup size
+std::vector<int> mix_variant_candidates(int K) {
+  if (K == 1) return {25632, 25616, 51216, 102408};
+  if (K == 2) return {102404, 25608, 102408, 25616, 51216};
+  if (K == 4) return {25608, 25604, 102404};
+  return {25604, 51204};
 }
 
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {

@@ -92,6 +92,47 @@ def test_sigma_block_interleaved(pair, k, prec):
         assert rel <= tol, (v, rel)
 
 
+VARIANTS = {1: [25632, 25616, 51216, 102408], 2: [102404, 25608, 102408, 25616, 51216],
+            4: [25608, 25604, 102404], 8: [25604, 51204]}
+
+
+@pytest.mark.parametrize("k", [1, 2, 4, 8])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_every_mixed_kernel_variant(pair, k, prec):
+    """Each variant the autotuner may pick (threads per workgroup x entries per thread of the
+    alpha-single x beta-single kernel) gives the oracle's sigma; then the autotuner's own
+    choice does, and variant 0 restores the default."""
+    torch = pytest.importorskip("torch")
+    name, inst, so, eng = pair
+    if name != "c2fam_300x280" and not (name == "l10_3a3b_samp" and k <= 2):
+        pytest.skip("variants checked on two cases")
+    X = np.random.default_rng(20 + k).standard_normal((so.ndet, k))
+    refs = [so.mult(X[:, v]) for v in range(k)]
+    dt = torch.float64 if prec == 0 else torch.float32
+    dx = torch.tensor(X, dtype=dt, device="cuda").contiguous()
+    tol = 1e-12 if prec == 0 else 2e-6
+
+    def check_now(tag):
+        dy = torch.empty_like(dx)
+        eng.sigma_block_device(prec, k, dx.data_ptr(), dy.data_ptr())
+        Y = dy.cpu().numpy().astype(np.float64).reshape(so.ndet, k)
+        for v in range(k):
+            rel = np.max(np.abs(Y[:, v] - refs[v])) / np.max(np.abs(refs[v]))
+            assert rel <= tol, (tag, v, rel)
+
+    for var in VARIANTS[k]:
+        print(f"[variant {var} k={k} prec={prec}]", flush=True)
+        eng.set_mix_variant(prec, k, var)
+        check_now(var)
+    chosen, cand = eng.autotune(prec, k, reps=1)
+    assert sorted(cand) == sorted(VARIANTS[k]) and chosen in cand
+    check_now("autotuned")
+    eng.set_mix_variant(prec, k, 0)
+    check_now("default")
+    with pytest.raises(SbdxError):
+        eng.set_mix_variant(prec, k, 25633)
+
+
 def test_davidson_energy(pair):
     name, inst, so, eng = pair
     if so.ndet > 20000:
