@@ -18,8 +18,8 @@
 //   alpha single ja -> ia (i -> a):  s_a (F_a + sum_{j in beta(ib)} (ai|jj))
 //   alpha double:                     stored value (sign and exchange included)
 //   alpha single x beta single:       s_a s_b (ai|bj)
-// Arithmetic: FP64 (FMA) or FP32 with compensated accumulation (error-free product and sum
-// transformations, i.e. Ogita-Rump-Oishi Dot2 per output element).
+// Arithmetic: FP64 (FMA), or FP32 data whose products (exact in FP64) are accumulated in FP64
+// per output element and rounded to FP32 once.
 #include "common.h"
 
 #include <vector>
