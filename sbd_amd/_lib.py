@@ -305,12 +305,12 @@ class Engine:
 
     def autotune(self, prec, k, reps=2):
         """Time the mixed-pass variants for (prec, k) and keep the fastest; returns
-        (chosen variant, {variant: mean ms}), variant = 100 * threads per workgroup + entries
-        per thread."""
-        cand = np.zeros(8, np.int32)
-        ms = np.zeros(8)
+        (chosen variant, {variant: mean ms}), variant = 1000000 (stages - 1) + 100 * threads
+        per workgroup + entries per thread."""
+        cand = np.zeros(16, np.int32)
+        ms = np.zeros(16)
         chosen, nc = C.c_int(), C.c_int()
-        check(lib().sbdx_autotune(self._h, prec, k, reps, C.byref(chosen), cand, ms, 8, C.byref(nc)))
+        check(lib().sbdx_autotune(self._h, prec, k, reps, C.byref(chosen), cand, ms, len(cand), C.byref(nc)))
         return chosen.value, {int(cand[i]): float(ms[i]) for i in range(nc.value)}
 
     def set_mix_variant(self, prec, k, variant):

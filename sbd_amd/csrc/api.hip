@@ -403,7 +403,9 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
     hipEvent_t e0, e1;
     SBDX_HIP(hipEventCreate(&e0));
     SBDX_HIP(hipEventCreate(&e1));
-    const std::vector<int> cand = sbdx::mix_variant_candidates(k);
+    std::vector<int> cand;
+    for (int v : sbdx::mix_variant_candidates(k))
+      if (sbdx::mix_variant_fits(*c, prec, k, v)) cand.push_back(v);
     int& slot = c->mix_variant_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)];
     int best = 0;
     double best_ms = 0.0;
@@ -447,6 +449,8 @@ int sbdx_set_mix_variant(sbdx_ctx* c, int prec, int k, int variant) {
     const std::vector<int> cand = sbdx::mix_variant_candidates(k);
     need(variant == 0 || std::find(cand.begin(), cand.end(), variant) != cand.end(),
          "not a mixed-kernel variant for this block width");
+    need(variant == 0 || sbdx::mix_variant_fits(*c, prec, k, variant),
+         "this mixed-kernel variant needs more LDS than a CU has for this problem");
     c->mix_variant_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)] = variant;
   });
 }

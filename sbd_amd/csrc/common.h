@@ -224,6 +224,7 @@ void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
 void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
 void reset_mix_tiles(sbdx_ctx& c);              // sigma.hip: drop cached mixed-pass tiles
 std::vector<int> mix_variant_candidates(int k); // sigma.hip: autotuner search space
+bool mix_variant_fits(const sbdx_ctx& c, int prec, int k, int variant);  // its LDS fits a CU
 // sigma of nvec vectors stored one after another (vector-major)
 void sigma_blocked(sbdx_ctx& c, int prec, int nvec, const void* x, void* y, hipStream_t s);
 // fused sigma of a block of k vectors stored interleaved, element [det * k + v], k in {1,2,4,8}

@@ -178,15 +178,19 @@ __global__ __launch_bounds__(kTile) void k_sigma_rows(
 // share that row's ket rows in L2.  Per-entry sums are folded into the tile's columns with LDS
 // atomics once at the end; sigma is updated by its owner, with no global atomics.
 constexpr int kMixMaxCols = 1024;
-// Variants of the kernel: entries per thread (kEPT) and threads per workgroup (kMixThreads),
-// coded as 100 * threads + entries per thread.  Every workgroup of a row gathers from all of
+constexpr size_t kMixMaxLds = 160 * 1024;  // LDS of one CU
+// Variants of the kernel: entries per thread (kEPT), threads per workgroup (kMixThreads) and
+// alpha singles staged per barrier (kStage), coded as 1000000 (kStage - 1) + 100 * threads +
+// entries per thread.  Every workgroup of a row gathers from all of
 // the row's ket rows, touching most of their cache lines whatever its tile size, so larger
 // tiles (threads x entries) mean fewer line fetches per row; more entries per thread mean
 // more registers and fewer waves.  The autotuner picks among them per (precision, K).
 // V row elements fetched per thread: kVPerThread * kMixThreads covers L^2 (2048: L <= 45,
-// 4096: L <= 64)
+// 4096: L <= 64).  kStage alpha singles are staged per barrier (their V rows in LDS, their
+// gathers in flight together); the sums still take the alpha singles in ascending order, so
+// the result does not depend on kStage.
 
-template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads>
+template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage>
 __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     int64_t na, int64_t nb, int L, const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket,
     const uint32_t* __restrict__ a_exc, int ntile, const int64_t* __restrict__ t_off,
@@ -195,9 +199,8 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     const T* __restrict__ x, T* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L2 = L * L;
-  T* vs0 = reinterpret_cast<T*>(smem_raw);
-  T* vs1 = vs0 + L2;
-  T* out = vs1 + L2;
+  T* vs = reinterpret_cast<T*>(smem_raw);  // [2][kStage][L2]: double-buffered V rows
+  T* out = vs + 2 * kStage * L2;
 
   // XCD-aware numbering: workgroup g runs on XCD g % 8; XCD x takes rows ia = x (mod 8), all
   // tiles of a row back to back
@@ -226,46 +229,63 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
   }
   for (int c = tid; c < ncol * K; c += kMixThreads) out[c] = T(0);
 
-  // V row of the next alpha single: fetched into registers before the current row's gathers
-  // (its latency hides under them), written to LDS after them
-  T pre[kVPerThread];
-  bool pre_neg = false;
-  auto fetch = [&](int64_t k) {
-    const uint32_t e = a_exc[k];
-    const T* src = V + static_cast<int64_t>(((e >> 6) & 63) * L + (e & 63)) * L2;
-    pre_neg = (e >> 12) & 1u;
+  // V rows of the next stage's alpha singles: fetched into registers before the current
+  // stage's gathers (their latency hides under them), written to LDS after them
+  T pre[kStage][kVPerThread];
+  bool pre_neg[kStage];
+  auto fetch = [&](int64_t kk) {
 #pragma unroll
-    for (int u = 0; u < kVPerThread; ++u) {
-      const int p = tid + u * kMixThreads;
-      pre[u] = src[p < L2 ? p : 0];
+    for (int q = 0; q < kStage; ++q) {
+      if (kk + q >= k1) break;
+      const uint32_t e = a_exc[kk + q];
+      const T* src = V + static_cast<int64_t>(((e >> 6) & 63) * L + (e & 63)) * L2;
+      pre_neg[q] = (e >> 12) & 1u;
+#pragma unroll
+      for (int u = 0; u < kVPerThread; ++u) {
+        const int p = tid + u * kMixThreads;
+        pre[q][u] = src[p < L2 ? p : 0];
+      }
     }
   };
-  auto commit = [&](T* dst) {
+  auto commit = [&](T* dst, int64_t kk) {
 #pragma unroll
-    for (int u = 0; u < kVPerThread; ++u) {
-      const int p = tid + u * kMixThreads;
-      if (p < L2) dst[p] = pre_neg ? -pre[u] : pre[u];
+    for (int q = 0; q < kStage; ++q) {
+      if (kk + q >= k1) break;
+#pragma unroll
+      for (int u = 0; u < kVPerThread; ++u) {
+        const int p = tid + u * kMixThreads;
+        if (p < L2) dst[q * L2 + p] = pre_neg[q] ? -pre[q][u] : pre[q][u];
+      }
     }
   };
   fetch(k0);
-  commit(vs0);
-  for (int64_t k = k0; k < k1; ++k) {
-    T* cur = ((k - k0) & 1) ? vs1 : vs0;
-    T* nxt = ((k - k0) & 1) ? vs0 : vs1;
+  commit(vs, k0);
+  int buf = 0;
+  for (int64_t kk = k0; kk < k1; kk += kStage, buf ^= 1) {
+    T* cur = vs + buf * kStage * L2;
+    T* nxt = vs + (buf ^ 1) * kStage * L2;
     __syncthreads();  // cur is staged; nobody reads nxt any more
-    const bool more = k + 1 < k1;
-    if (more) fetch(k + 1);
-    const T* __restrict__ xr = x + static_cast<int64_t>(a_ket[k]) * nb * K;
-    VecK<T, K> xv[kEPT];
+    const bool more = kk + kStage < k1;
+    if (more) fetch(kk + kStage);
+    VecK<T, K> xv[kStage][kEPT];
 #pragma unroll
-    for (int j = 0; j < kEPT; ++j) xv[j] = ldk<T, K>(xr + static_cast<int64_t>(jb[j]) * K);
+    for (int q = 0; q < kStage; ++q) {
+      if (kk + q >= k1) break;
+      const T* __restrict__ xr = x + static_cast<int64_t>(a_ket[kk + q]) * nb * K;
 #pragma unroll
-    for (int j = 0; j < kEPT; ++j) {
-      const T w = cur[(meta[j] >> 16) & 0xfff];
-#pragma unroll
-      for (int v = 0; v < K; ++v) acc[j][v].fma(w, xv[j].v[v]);
+      for (int j = 0; j < kEPT; ++j) xv[q][j] = ldk<T, K>(xr + static_cast<int64_t>(jb[j]) * K);
     }
-    if (more) commit(nxt);
+#pragma unroll
+    for (int q = 0; q < kStage; ++q) {
+      if (kk + q >= k1) break;
+#pragma unroll
+      for (int j = 0; j < kEPT; ++j) {
+        const T w = cur[q * L2 + ((meta[j] >> 16) & 0xfff)];
+#pragma unroll
+        for (int v = 0; v < K; ++v) acc[j][v].fma(w, xv[q][j].v[v]);
+      }
+    }
+    if (more) commit(nxt, kk + kStage);
   }
   __syncthreads();
 #pragma unroll
@@ -354,9 +374,9 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
 int mix_variant(const sbdx_ctx& c, int prec, int K) {
   const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
   if (tuned) return tuned;
-  if (K == 1) return prec == 0 ? 51216 : 25632;
-  if (K == 2) return prec == 0 ? 51216 : 102404;
-  if (K == 4) return prec == 0 ? 102404 : 25608;
+  if (K == 1) return prec == 0 ? 1102408 : 25632;
+  if (K == 2) return prec == 0 ? 51216 : 1051208;
+  if (K == 4) return prec == 0 ? 1102404 : 25608;
   return 25604;
 }
 
@@ -433,10 +453,12 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
 template <typename T, int K>
 void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   const int var = mix_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
-  const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * (var / 100));
+  const int stages = var / 1000000 + 1, threads = (var % 1000000) / 100;
+  const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
   if (mt.ntile == 0) return;
   const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
-  const size_t lds = (2 * static_cast<size_t>(c.L) * c.L + static_cast<size_t>(mt.max_cols) * K) * sizeof(T);
+  const size_t lds = (2 * static_cast<size_t>(stages) * c.L * c.L + static_cast<size_t>(mt.max_cols) * K) * sizeof(T);
+  if (lds > kMixMaxLds) throw Error(SBDX_ERR_ARG, "mixed-kernel variant needs more LDS than a CU has for this problem");
   auto launch = [&](auto kernel, int threads) {
     if (lds > 65536)
       SBDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
@@ -447,9 +469,10 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
     SBDX_CHECK_LAUNCH();
   };
   const bool small_v = static_cast<int>(c.L) * c.L <= 2048;
-  auto pick = [&](auto e, auto t) {
-    constexpr int E = decltype(e)::value, NT = decltype(t)::value;
-    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT>, NT) : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT>, NT);
+  auto pick = [&](auto e, auto t, auto sg) {
+    constexpr int E = decltype(e)::value, NT = decltype(t)::value, SG = decltype(sg)::value;
+    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG>, NT)
+            : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG>, NT);
   };
   using std::integral_constant;
   using I4 = integral_constant<int, 4>;
@@ -459,28 +482,37 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   using T256 = integral_constant<int, 256>;
   using T512 = integral_constant<int, 512>;
   using T1024 = integral_constant<int, 1024>;
+  using S1 = integral_constant<int, 1>;
+  using S2 = integral_constant<int, 2>;
+  using S4 = integral_constant<int, 4>;
   // the candidates of mix_variant_candidates(K); anything else is refused
   if constexpr (K == 1) {
-    if (var == 25632) pick(I32{}, T256{});
-    else if (var == 25616) pick(I16{}, T256{});
-    else if (var == 51216) pick(I16{}, T512{});
-    else if (var == 102408) pick(I8{}, T1024{});
+    if (var == 25632) pick(I32{}, T256{}, S1{});
+    else if (var == 25616) pick(I16{}, T256{}, S1{});
+    else if (var == 51216) pick(I16{}, T512{}, S1{});
+    else if (var == 102408) pick(I8{}, T1024{}, S1{});
+    else if (var == 1102408) pick(I8{}, T1024{}, S2{});
+    else if (var == 1051216) pick(I16{}, T512{}, S2{});
     else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   } else if constexpr (K == 2) {
-    if (var == 102404) pick(I4{}, T1024{});
-    else if (var == 25608) pick(I8{}, T256{});
-    else if (var == 102408) pick(I8{}, T1024{});
-    else if (var == 25616) pick(I16{}, T256{});
-    else if (var == 51216) pick(I16{}, T512{});
+    if (var == 102404) pick(I4{}, T1024{}, S1{});
+    else if (var == 25608) pick(I8{}, T256{}, S1{});
+    else if (var == 102408) pick(I8{}, T1024{}, S1{});
+    else if (var == 25616) pick(I16{}, T256{}, S1{});
+    else if (var == 51216) pick(I16{}, T512{}, S1{});
+    else if (var == 1102404) pick(I4{}, T1024{}, S2{});
+    else if (var == 3102404) pick(I4{}, T1024{}, S4{});
+    else if (var == 1051208) pick(I8{}, T512{}, S2{});
     else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   } else if constexpr (K == 4) {
-    if (var == 25608) pick(I8{}, T256{});
-    else if (var == 25604) pick(I4{}, T256{});
-    else if (var == 102404) pick(I4{}, T1024{});
+    if (var == 25608) pick(I8{}, T256{}, S1{});
+    else if (var == 25604) pick(I4{}, T256{}, S1{});
+    else if (var == 102404) pick(I4{}, T1024{}, S1{});
+    else if (var == 1102404) pick(I4{}, T1024{}, S2{});
     else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   } else {
-    if (var == 25604) pick(I4{}, T256{});
-    else if (var == 51204) pick(I4{}, T512{});
+    if (var == 25604) pick(I4{}, T256{}, S1{});
+    else if (var == 51204) pick(I4{}, T512{}, S1{});
     else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   }
 }
@@ -571,13 +603,20 @@ void reset_mix_tiles(sbdx_ctx& c) {
   for (auto& m : c.mix) m = MixTiles{};
 }
 
-// variants (100 * threads + entries per thread) of the mixed kernel for block width K: none
-// spills at its workgroup size
+// variants (1000000 (stages - 1) + 100 threads + entries per thread) of the mixed kernel for
+// block width K
 std::vector<int> mix_variant_candidates(int K) {
-  if (K == 1) return {25632, 25616, 51216, 102408};
-  if (K == 2) return {102404, 25608, 102408, 25616, 51216};
-  if (K == 4) return {25608, 25604, 102404};
+  if (K == 1) return {25632, 25616, 51216, 102408, 1102408, 1051216};
+  if (K == 2) return {102404, 25608, 102408, 25616, 51216, 1102404, 3102404, 1051208};
+  if (K == 4) return {25608, 25604, 102404, 1102404};
   return {25604, 51204};
+}
+
+// whether a variant's LDS (double-buffered V rows of its stages + the tile's columns, at most
+// kMixMaxCols) fits a CU for this problem
+bool mix_variant_fits(const sbdx_ctx& c, int prec, int K, int var) {
+  const size_t stages = var / 1000000 + 1, es = prec == 0 ? 8 : 4;
+  return (2 * stages * c.L * c.L + static_cast<size_t>(kMixMaxCols) * K) * es <= kMixMaxLds;
 }
 
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {

@@ -92,8 +92,9 @@ def test_sigma_block_interleaved(pair, k, prec):
         assert rel <= tol, (v, rel)
 
 
-VARIANTS = {1: [25632, 25616, 51216, 102408], 2: [102404, 25608, 102408, 25616, 51216],
-            4: [25608, 25604, 102404], 8: [25604, 51204]}
+VARIANTS = {1: [25632, 25616, 51216, 102408, 1102408, 1051216],
+            2: [102404, 25608, 102408, 25616, 51216, 1102404, 3102404, 1051208],
+            4: [25608, 25604, 102404, 1102404], 8: [25604, 51204]}
 
 
 @pytest.mark.parametrize("k", [1, 2, 4, 8])
