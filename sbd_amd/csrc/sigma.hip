@@ -388,7 +388,10 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
     if (m.cap == cap && m.ent_off.get()) return m;
     if (!slot && m.cap == 0) slot = &m;
   }
-  MixTiles& mt = slot ? *slot : c.mix[0];
+  // tile sets are never evicted: captured graphs (batches) hold their addresses.  The
+  // variants use four tile sizes (1024 to 8192 entries), fewer than the slots.
+  if (!slot) throw Error(SBDX_ERR_STATE, "no free slot for another mixed-pass tile size");
+  MixTiles& mt = *slot;
   mt = MixTiles{};
   mt.cap = cap;
   const SpinTab& B = c.B;
