@@ -206,6 +206,9 @@ def main():
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--strings", type=int, default=None, help="reduced string count (tests)")
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--fp32-acc", default="fp64", choices=["fp64", "twosum"],
+                    help="accumulation of FP32 sigma builds: exact products summed in FP64, or "
+                         "compensated FP32 partials (two-sum)")
     ap.add_argument("--solve", nargs="*", default=None, choices=sorted(SOLVERS),
                     help="also time these Davidson variants to 1e-6 Ha of the reference energy")
     ap.add_argument("--solve-time", type=float, default=240.0)
@@ -247,6 +250,7 @@ def main():
     torch.cuda.synchronize()
     t = time.perf_counter()
     eng = Engine.from_instance(inst)
+    eng.set_fp32_accumulation(1 if args.fp32_acc == "twosum" else 0)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t
     parts = nnz_parts(eng)
@@ -321,6 +325,7 @@ def main():
             "ndet": N,
             "offdiag_elements": nnz,
             "precision": args.prec,
+            "fp32_accumulation": args.fp32_acc,
         },
         "roofline": {
             "kernel": PHASE_KERNELS[dom],
@@ -351,7 +356,7 @@ def main():
         out["block_sigma"] = {}
         for k in args.block:
             tuned = None
-            if args.autotune:
+            if args.autotune and not (prec == SBDX_FP32 and args.fp32_acc == "twosum"):
                 chosen, cand = eng.autotune(prec, k, reps=1)
                 tuned = {"variant": chosen, "candidates_ms": cand}
                 log(f"[bench] autotune block {k}: {tuned}")

@@ -45,6 +45,13 @@ typedef enum {
 /* Arithmetic of a sigma build. */
 enum { SBDX_FP64 = 0, SBDX_FP32 = 1 };
 
+/* Accumulation of FP32 sigma builds (FP32 integrals and vectors either way).
+ * SBDX_ACC_FP64: each product of two FP32 numbers is exact in FP64 and is added unrounded into
+ *   an FP64 accumulator, rounded to FP32 once per output (the default; fewer instructions).
+ * SBDX_ACC_TWOSUM: FP32 accumulation with compensated partials, an error-free product (FMA
+ *   residual) and a TwoSum per term (Ogita-Rump-Oishi Dot2). */
+enum { SBDX_ACC_FP64 = 0, SBDX_ACC_TWOSUM = 1 };
+
 typedef struct sbdx_ctx sbdx_ctx;
 
 /* Create a context on the current HIP device.  alpha/beta: n_alpha/n_beta uint64 strings (any
@@ -242,6 +249,11 @@ void sbdx_batch_destroy(sbdx_batch *batch);
  * the variant kept; *ncand: variants tried. */
 int sbdx_autotune(sbdx_ctx *ctx, int prec, int k, int reps, int *chosen, int *candidates,
                   double *ms, int cap, int *ncand);
+
+/* No counterpart in the reference: the accumulation of this context's FP32 sigma builds
+ * (SBDX_ACC_FP64 or SBDX_ACC_TWOSUM; SBDX_ERR_ARG otherwise).  Two-sum builds use fixed
+ * kernel variants: sbdx_autotune and sbdx_set_mix_variant act on the default accumulation. */
+int sbdx_set_fp32_accumulation(sbdx_ctx *ctx, int mode);
 
 /* No counterpart in the reference: pin the alpha-single x beta-single kernel variant for
  * (prec, k) to one of the autotuner's candidates (the values sbdx_autotune reports), or back to

@@ -22,7 +22,7 @@ EXPORTS = [
     "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
     "sbdx_profile_read", "sbdx_destroy", "sbdx_last_error",
     "sbdx_mlp_train_defaults", "sbdx_mlp_record", "sbdx_mlp_loss_grad", "sbdx_mlp_forward",
-    "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune", "sbdx_set_mix_variant",
+    "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune", "sbdx_set_mix_variant", "sbdx_set_fp32_accumulation",
     "sbdx_batch_create", "sbdx_batch_davidson", "sbdx_batch_destroy", "sbdx_store_hamiltonian",
 ]
 MLP_NPARAM = 12609
@@ -110,6 +110,7 @@ def lib():
     L.sbdx_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), i32p, f64p,
                                 C.c_int, C.POINTER(C.c_int)]
     L.sbdx_set_mix_variant.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.sbdx_set_fp32_accumulation.argtypes = [vp, C.c_int]
     L.sbdx_batch_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp)]
     L.sbdx_batch_davidson.argtypes = [vp, C.POINTER(DavidsonOpts), C.POINTER(vp),
                                       C.POINTER(DavidsonResult)]
@@ -312,6 +313,10 @@ class Engine:
         chosen, nc = C.c_int(), C.c_int()
         check(lib().sbdx_autotune(self._h, prec, k, reps, C.byref(chosen), cand, ms, len(cand), C.byref(nc)))
         return chosen.value, {int(cand[i]): float(ms[i]) for i in range(nc.value)}
+
+    def set_fp32_accumulation(self, mode):
+        """0: exact FP32 products accumulated in FP64 (default); 1: compensated FP32 (two-sum)."""
+        check(lib().sbdx_set_fp32_accumulation(self._h, int(mode)))
 
     def set_mix_variant(self, prec, k, variant):
         """Pin the mixed-pass variant for (prec, k) (one of autotune's candidates; 0: default)."""

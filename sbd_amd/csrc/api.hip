@@ -396,6 +396,8 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
     need_ints(c);
     need((k == 1 || k == 2 || k == 4 || k == 8) && (prec == SBDX_FP64 || prec == SBDX_FP32) && reps >= 1,
          "bad argument (k must be 1, 2, 4 or 8)");
+    need(prec == SBDX_FP64 || c->fp32_acc == SBDX_ACC_FP64,
+         "the autotuner tunes the default FP32 accumulation; two-sum builds use fixed variants");
     const int64_t n = c->ndet();
     const size_t es = prec == SBDX_FP64 ? 8 : 4;
     sbdx::DevBuf<unsigned char> x(n * k * es), y(n * k * es);
@@ -438,6 +440,14 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
     (void)hipEventDestroy(e1);
     if (chosen) *chosen = best;
     if (ncand) *ncand = static_cast<int>(cand.size());
+  });
+}
+
+int sbdx_set_fp32_accumulation(sbdx_ctx* c, int mode) {
+  return guarded([&] {
+    need(c != nullptr, "null context");
+    need(mode == SBDX_ACC_FP64 || mode == SBDX_ACC_TWOSUM, "mode must be SBDX_ACC_FP64 or SBDX_ACC_TWOSUM");
+    c->fp32_acc = mode;
   });
 }
 

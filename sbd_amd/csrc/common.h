@@ -133,6 +133,7 @@ struct sbdx_ctx {
   sbdx::Workspace ws;
   sbdx::Prof prof;
   int mix_variant_tuned[2][4] = {}; // autotuned mixed-kernel variant [FP64|FP32][K slot]
+  int fp32_acc = SBDX_ACC_FP64;     // accumulation of FP32 sigma builds (sbdx_set_fp32_accumulation)
   bool stored = false;             // H stored as CSR (stored.hip): sigma builds use it
   sbdx::DevBuf<int64_t> h_rowptr;
   sbdx::DevBuf<uint32_t> h_col;

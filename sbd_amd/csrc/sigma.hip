@@ -19,7 +19,8 @@
 //   alpha double:                     stored value (sign and exchange included)
 //   alpha single x beta single:       s_a s_b (ai|bj)
 // Arithmetic: FP64 (FMA), or FP32 data whose products (exact in FP64) are accumulated in FP64
-// per output element and rounded to FP32 once.
+// per output element and rounded to FP32 once, or (SBDX_ACC_TWOSUM) FP32 data accumulated in
+// FP32 with compensated partials (error-free product + TwoSum per term).
 #include "common.h"
 
 #include <vector>
@@ -52,9 +53,29 @@ struct AccX {
   __device__ inline float get() const { return static_cast<float>(s); }
 };
 
-template <typename T> struct AccOf;
-template <> struct AccOf<double> { using type = AccD; };
-template <> struct AccOf<float> { using type = AccX; };
+// compensated FP32 (SBDX_ACC_TWOSUM): p = a*b exactly as p + e (FMA residual), s + p exactly
+// as t + err (TwoSum), errors gathered in c (Ogita-Rump-Oishi Dot2)
+struct AccF {
+  float s = 0.0f, c = 0.0f;
+  __device__ inline void add(float p) {
+    const float t = __fadd_rn(s, p);
+    const float bp = __fsub_rn(t, s);
+    c += __fadd_rn(__fsub_rn(s, __fsub_rn(t, bp)), __fsub_rn(p, bp));
+    s = t;
+  }
+  __device__ inline void fma(float a, float b) {
+    const float p = __fmul_rn(a, b);
+    c += __fmaf_rn(a, b, -p);
+    add(p);
+  }
+  __device__ inline float get() const { return s + c; }
+};
+
+// accumulator of a precision and FP32 accumulation mode
+template <typename T, int kAcc> struct AccOf;
+template <int kAcc> struct AccOf<double, kAcc> { using type = AccD; };
+template <> struct AccOf<float, SBDX_ACC_FP64> { using type = AccX; };
+template <> struct AccOf<float, SBDX_ACC_TWOSUM> { using type = AccF; };
 
 // K contiguous values, loaded and stored as whole 16-byte words where possible
 template <typename T, int K>
@@ -84,7 +105,7 @@ struct RowTabs {
 // singles and doubles of value * x[ket, col], for K interleaved vectors.  One workgroup = one
 // row x kTile columns: the row's list entries are the same for every lane (scalar loads), the
 // gathers of one entry are contiguous across the lanes.
-template <typename T, bool kDiag, int K>
+template <typename T, bool kDiag, int K, int kAcc>
 __global__ __launch_bounds__(kTile) void k_sigma_rows(
     int64_t nrow, int64_t ncol, int L, RowTabs rt, const T* __restrict__ val_s,
     const T* __restrict__ val_d, const uint64_t* __restrict__ col_str, const T* __restrict__ V,
@@ -97,7 +118,7 @@ __global__ __launch_bounds__(kTile) void k_sigma_rows(
   const int64_t col = tile * kTile + threadIdx.x;
   if (col >= ncol) return;
   const int64_t L2 = static_cast<int64_t>(L) * L;
-  typename AccOf<T>::type acc[K];
+  typename AccOf<T, kAcc>::type acc[K];
   const int64_t me = row * ncol + col;
   if (kDiag) {
     const VecK<T, K> xm = ldk<T, K>(x + me * K);
@@ -190,7 +211,7 @@ constexpr size_t kMixMaxLds = 160 * 1024;  // LDS of one CU
 // gathers in flight together); the sums still take the alpha singles in ascending order, so
 // the result does not depend on kStage.
 
-template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage>
+template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage, int kAcc>
 __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     int64_t na, int64_t nb, int L, const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket,
     const uint32_t* __restrict__ a_exc, int ntile, const int64_t* __restrict__ t_off,
@@ -216,7 +237,7 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
   const int64_t e0 = t_off[tile], e1 = t_off[tile + 1];
   const int col0 = t_col0[tile], ncol = t_ncol[tile];
   uint32_t jb[kEPT], meta[kEPT];
-  typename AccOf<T>::type acc[kEPT][K];
+  typename AccOf<T, kAcc>::type acc[kEPT][K];
   // padding entries (past the tile's end) read column 0 and pair 0, which are always valid, so
   // the entry loop below has no branches and all its gathers can be in flight at once; their
   // sums are dropped at the end (ib_local 0xffff)
@@ -453,9 +474,14 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   return mt;
 }
 
-template <typename T, int K>
+// the variant of the two-sum FP32 kernels: the autotuner's picks at c2 when two-sum was the
+// only FP32 accumulation (the variants are tuned for the default accumulation)
+constexpr int mix_twosum_variant(int K) { return K == 1 ? 25632 : (K == 8 ? 25604 : 25608); }
+
+template <typename T, int K, int kAcc>
 void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
-  const int var = mix_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
+  const int var = kAcc == SBDX_ACC_TWOSUM ? mix_twosum_variant(K)
+                                          : mix_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
   const int stages = var / 1000000 + 1, threads = (var % 1000000) / 100;
   const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
   if (mt.ntile == 0) return;
@@ -474,8 +500,8 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   const bool small_v = static_cast<int>(c.L) * c.L <= 2048;
   auto pick = [&](auto e, auto t, auto sg) {
     constexpr int E = decltype(e)::value, NT = decltype(t)::value, SG = decltype(sg)::value;
-    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG>, NT)
-            : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG>, NT);
+    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc>, NT)
+            : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc>, NT);
   };
   using std::integral_constant;
   using I4 = integral_constant<int, 4>;
@@ -489,7 +515,11 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   using S2 = integral_constant<int, 2>;
   using S4 = integral_constant<int, 4>;
   // the candidates of mix_variant_candidates(K); anything else is refused
-  if constexpr (K == 1) {
+  if constexpr (kAcc == SBDX_ACC_TWOSUM) {
+    if constexpr (K == 1) pick(I32{}, T256{}, S1{});
+    else if constexpr (K == 8) pick(I4{}, T256{}, S1{});
+    else pick(I8{}, T256{}, S1{});
+  } else if constexpr (K == 1) {
     if (var == 25632) pick(I32{}, T256{}, S1{});
     else if (var == 25616) pick(I16{}, T256{}, S1{});
     else if (var == 51216) pick(I16{}, T512{}, S1{});
@@ -520,7 +550,7 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   }
 }
 
-template <typename T, int K>
+template <typename T, int K, int kAcc>
 void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const T* vd_a,
              const T* vs_b, const T* vd_b, const T* V, const T* hii) {
   const int64_t na = c.na, nb = c.nb;
@@ -533,12 +563,12 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   };
   mark(0);
   // alpha rows: diagonal + alpha singles and doubles
-  k_sigma_rows<T, true, K><<<static_cast<unsigned>(na * cdiv(nb, kTile)), kTile, 0, st>>>(
+  k_sigma_rows<T, true, K, kAcc><<<static_cast<unsigned>(na * cdiv(nb, kTile)), kTile, 0, st>>>(
       na, nb, c.L, ta, vs_a, vd_a, c.B.str.get(), V, x, y, hii);
   SBDX_CHECK_LAUNCH();
   mark(1);
   // alpha single x beta single
-  launch_mixed<T, K>(c, x, y, V, st);
+  launch_mixed<T, K, kAcc>(c, x, y, V, st);
   mark(2);
   // beta rows on the transposed vector
   c.ws.xT.ensure(n * K * sizeof(T));
@@ -550,7 +580,7 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   k_transpose<T, K><<<tblocks, 256, 0, st>>>(x, xT, na, nb);
   SBDX_CHECK_LAUNCH();
   mark(3);
-  k_sigma_rows<T, false, K><<<static_cast<unsigned>(nb * cdiv(na, kTile)), kTile, 0, st>>>(
+  k_sigma_rows<T, false, K, kAcc><<<static_cast<unsigned>(nb * cdiv(na, kTile)), kTile, 0, st>>>(
       nb, na, c.L, tb, vs_b, vd_b, c.A.str.get(), V, xT, yT, nullptr);
   SBDX_CHECK_LAUNCH();
   mark(4);
@@ -578,13 +608,19 @@ void sigma_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
   const T* vd_b = f ? reinterpret_cast<const T*>(c.B.val_d32.get()) : reinterpret_cast<const T*>(c.B.val_d.get());
   const T* V = f ? reinterpret_cast<const T*>(c.eri4_32.get()) : reinterpret_cast<const T*>(c.eri4.get());
   const T* hii = f ? reinterpret_cast<const T*>(c.hii32.get()) : reinterpret_cast<const T*>(c.hii.get());
-  switch (k) {
-    case 1: sigma_k<T, 1>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
-    case 2: sigma_k<T, 2>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
-    case 4: sigma_k<T, 4>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
-    case 8: sigma_k<T, 8>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
-    default: throw Error(-1, "block width must be 1, 2, 4 or 8");
-  }
+  auto go = [&](auto acc) {
+    constexpr int A = decltype(acc)::value;
+    switch (k) {
+      case 1: sigma_k<T, 1, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+      case 2: sigma_k<T, 2, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+      case 4: sigma_k<T, 4, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+      case 8: sigma_k<T, 8, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+      default: throw Error(-1, "block width must be 1, 2, 4 or 8");
+    }
+  };
+  // FP64 has one accumulation; FP32 follows the context's mode
+  if (f && c.fp32_acc == SBDX_ACC_TWOSUM) go(std::integral_constant<int, SBDX_ACC_TWOSUM>{});
+  else go(std::integral_constant<int, SBDX_ACC_FP64>{});
 }
 
 template <typename T>

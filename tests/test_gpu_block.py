@@ -83,6 +83,18 @@ def test_mixed_precision_refines_to_fp64(small):
     assert residuals(H, W, [evals[0]])[0] < 1e-8
 
 
+def test_mixed_precision_twosum_refines_to_fp64(small):
+    """The mixed solver with compensated FP32 (two-sum) sigma builds reaches the FP64 energy."""
+    name, so, eng, H, evals = small
+    eng.set_fp32_accumulation(1)
+    try:
+        r = eng.davidson(method=1, precision=1, block=2, num_block=8, max_iteration=300, eps=1e-9)
+    finally:
+        eng.set_fp32_accumulation(0)
+    assert r["converged"], r
+    assert abs(r["energy"] - evals[0]) < 1e-9
+
+
 def test_time_to_reference_energy(small):
     name, so, eng, H, evals = small
     r = eng.davidson(method=1, precision=1, block=2, num_block=8, max_iteration=300, eps=1e-12,

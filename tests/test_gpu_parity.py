@@ -134,6 +134,31 @@ def test_every_mixed_kernel_variant(pair, k, prec):
         eng.set_mix_variant(prec, k, 25633)
 
 
+@pytest.mark.parametrize("k", [1, 2, 4, 8])
+def test_fp32_twosum_accumulation(pair, k):
+    """FP32 sigma with compensated FP32 partials (SBDX_ACC_TWOSUM) against the oracle, and back
+    to the default FP64 accumulation of the exact FP32 products."""
+    torch = pytest.importorskip("torch")
+    name, inst, so, eng = pair
+    X = np.random.default_rng(40 + k).standard_normal((so.ndet, k))
+    refs = [so.mult(X[:, v]) for v in range(k)]
+    dx = torch.tensor(X, dtype=torch.float32, device="cuda").contiguous()
+    for mode in (1, 0):
+        eng.set_fp32_accumulation(mode)
+        dy = torch.empty_like(dx)
+        eng.sigma_block_device(1, k, dx.data_ptr(), dy.data_ptr())
+        Y = dy.cpu().numpy().astype(np.float64).reshape(so.ndet, k)
+        for v in range(k):
+            rel = np.max(np.abs(Y[:, v] - refs[v])) / np.max(np.abs(refs[v]))
+            assert rel <= 2e-6, (mode, v, rel)
+    eng.set_fp32_accumulation(1)
+    with pytest.raises(SbdxError):
+        eng.autotune(1, k, reps=1)  # the tuner tunes the default accumulation only
+    eng.set_fp32_accumulation(0)
+    with pytest.raises(SbdxError):
+        eng.set_fp32_accumulation(2)
+
+
 def test_davidson_energy(pair):
     name, inst, so, eng = pair
     if so.ndet > 20000:
