@@ -393,12 +393,17 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
   stk<T, K>(y + k, a);
 }
 
-// variant of the mixed kernel (100 * threads + entries per thread) for a block width K: the
-// context's autotuned choice when there is one (sbdx_autotune), else the autotuner's choice on
-// the 4e8-determinant 36-orbital workload
+// variant of the mixed kernel for a block width K: the context's autotuned choice when there
+// is one (sbdx_autotune), else the autotuner's choice on the 4e8-determinant 36-orbital
+// workload, or for more than 45 orbitals (integral rows of more than 2048 values: the large
+// variants drop to one wave per SIMD) its choice on the 1e8-determinant 48-orbital workload
 int mix_variant(const sbdx_ctx& c, int prec, int K) {
   const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
   if (tuned) return tuned;
+  if (c.L * c.L > 2048) {
+    if (K == 1) return 25616;
+    if (K == 2) return 25608;
+  }
   if (K == 1) return prec == 0 ? 1102408 : 25632;
   if (K == 2) return prec == 0 ? 51216 : 1051208;
   if (K == 4) return prec == 0 ? 1102404 : 25608;
@@ -478,13 +483,17 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   return mt;
 }
 
-// the variant of the two-sum FP32 kernels: the autotuner's picks at c2 when two-sum was the
-// only FP32 accumulation (the variants are tuned for the default accumulation)
-constexpr int mix_twosum_variant(int K) { return K == 1 ? 25632 : (K == 8 ? 25604 : 25608); }
+// the variant of the two-sum FP32 kernels: the autotuner's picks when two-sum was the only
+// FP32 accumulation, for up to 45 orbitals and above (the variants are tuned for the default
+// accumulation)
+inline int mix_twosum_variant(const sbdx_ctx& c, int K) {
+  if (K == 1) return c.L * c.L > 2048 ? 25616 : 25632;
+  return K == 8 ? 25604 : 25608;
+}
 
 template <typename T, int K, int kAcc>
 void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
-  const int var = kAcc == SBDX_ACC_TWOSUM ? mix_twosum_variant(K)
+  const int var = kAcc == SBDX_ACC_TWOSUM ? mix_twosum_variant(c, K)
                                           : mix_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
   const int stages = var / 1000000 + 1, threads = (var % 1000000) / 100;
   const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
@@ -520,9 +529,14 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   using S4 = integral_constant<int, 4>;
   // the candidates of mix_variant_candidates(K); anything else is refused
   if constexpr (kAcc == SBDX_ACC_TWOSUM) {
-    if constexpr (K == 1) pick(I32{}, T256{}, S1{});
-    else if constexpr (K == 8) pick(I4{}, T256{}, S1{});
-    else pick(I8{}, T256{}, S1{});
+    if constexpr (K == 1) {
+      if (var == 25616) pick(I16{}, T256{}, S1{});
+      else pick(I32{}, T256{}, S1{});
+    } else if constexpr (K == 8) {
+      pick(I4{}, T256{}, S1{});
+    } else {
+      pick(I8{}, T256{}, S1{});
+    }
   } else if constexpr (K == 1) {
     if (var == 25632) pick(I32{}, T256{}, S1{});
     else if (var == 25616) pick(I16{}, T256{}, S1{});
