@@ -54,6 +54,8 @@ SOLVERS = {
     # the learned preconditioner trained on 36-orbital family instances (sbd_amd/mlp_train.py)
     "mixed_block2_nb16_mlp": dict(method=1, precision=1, block=2, num_block=16, preconditioner=1,
                                   mlp="sbd_amd/weights/c2_family.npy"),
+    "mixed_block2_nb24_mlp": dict(method=1, precision=1, block=2, num_block=24, preconditioner=1,
+                                  mlp="sbd_amd/weights/c2_family.npy"),
     # eight lowest roots in one block of eight
     "mixed_roots8": dict(method=1, precision=1, nroots=8, num_block=4),
 }
