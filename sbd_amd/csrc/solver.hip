@@ -90,7 +90,10 @@ __global__ __launch_bounds__(kB) void k_gram(const TA* __restrict__ A, const TB*
 // determinants of L_q[det K + a] R[det K + c], for NL left blocks (FP64 or FP32 each) and one
 // right block.  Consecutive lanes take consecutive determinants, so every block is read in
 // whole lines and the right block once for all NL left ones.  The products are HBM-bound
-// (NL K^2 flops per NL K + K values read), so they run on the vector ALUs.
+// (NL K^2 flops per NL K + K values read), so they run on the vector ALUs: a matrix-core
+// version (v_mfma_f64_16x16x4f64 over LDS-staged chunks) measured 1.4-3.6x slower, since the
+// FP64 shape contracts 4 determinants per instruction and a K-wide right block fills K of its
+// 16 output columns.
 struct LeftSet {
   const void* p[32];
   int dbl[32];  // 1: FP64 block, 0: FP32 block
