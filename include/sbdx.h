@@ -110,7 +110,8 @@ typedef struct {
   double max_time;        /* wall-clock budget in seconds */
   int init;               /* 0: W[0] = 1 (HF, reference init 0); 1: vector 0 of W as given */
   int precision;          /* SBDX_FP64, or SBDX_FP32 = mixed precision (method 1 only): FP32
-                             correction blocks and sigmas, FP64 anchor sigma, Rayleigh-Ritz,
+                             correction blocks and sigmas (accumulated as set by
+                             sbdx_set_fp32_accumulation), FP64 anchor sigma, Rayleigh-Ritz,
                              orthogonalization and residual */
   int nroots;             /* lowest roots returned and converged, 1..8 (> 1: method 1) */
   int preconditioner;     /* 0: diagonal (Jacobi); 1: learned (method 1; falls back to 0 for
