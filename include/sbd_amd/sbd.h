@@ -107,47 +107,51 @@ struct FCIDump {
   std::vector<std::tuple<double, int, int, int, int>> integrals;
 };
 
-// FCIDUMP text: a namelist header "&FCI NORB=.., NELEC=.., ..." closed by "&END" or "/",
-// then one "value i j k l" line per integral.
+// FCIDUMP text (framework/fcidump.h:27-95, read the same way so that a file gives the same
+// header and integral records):
+//  * blank lines and lines that start with '!' are skipped everywhere;
+//  * header lines run up to the first line containing "&END" (case as written), which is
+//    dropped whole; "&FCI" is cut out of a header line and the lines are joined with nothing
+//    between them;
+//  * the joined header splits at ','; an item holding '=' gives key (before the first '=')
+//    and value (after it), both trimmed of blanks and tabs; later items replace earlier ones;
+//    items without '=' (the further values of ORBSYM=1,1,...) are dropped;
+//  * every later line is one "value i j k l" record.  A line that does not start with five
+//    numbers is skipped (the reference stores it with indices it never set).
 inline FCIDump LoadFCIDump(const std::string& filename) {
   std::ifstream in(filename);
-  if (!in.is_open()) throw std::runtime_error("Failed to open file: " + filename);
+  if (!in.is_open()) throw std::runtime_error("Failed to open FCIDUMP file.");
   FCIDump d;
   std::string line, head;
   bool in_header = true;
   while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '!') continue;
     if (in_header) {
-      head += line + ",";
-      std::string up = line;
-      std::transform(up.begin(), up.end(), up.begin(), ::toupper);
-      if (up.find("&END") != std::string::npos || up.find('/') != std::string::npos) {
+      if (line.find("&END") != std::string::npos) {
         in_header = false;
-        std::string h = head;
-        for (char& ch : h)
-          if (ch == '&' || ch == '/') ch = ',';
-        std::stringstream ss(h);
-        std::string tok, key;
-        while (std::getline(ss, tok, ',')) {
-          auto eq = tok.find('=');
-          auto trim = [](std::string s) {
-            s.erase(0, s.find_first_not_of(" \t\r\n"));
-            s.erase(s.find_last_not_of(" \t\r\n") + 1);
-            return s;
-          };
-          if (eq != std::string::npos) {
-            key = trim(tok.substr(0, eq));
-            d.header[key] = trim(tok.substr(eq + 1));
-          } else if (!key.empty() && !trim(tok).empty() && trim(tok) != "FCI" && trim(tok) != "END") {
-            d.header[key] += "," + trim(tok);
-          }
-        }
+        continue;
       }
+      const size_t fci = line.find("&FCI");
+      if (fci != std::string::npos) line.erase(fci, 4);
+      head += line;
       continue;
     }
     std::istringstream ls(line);
-    double v;
-    int i, j, k, l;
+    double v = 0.0;
+    int i = 0, j = 0, k = 0, l = 0;
     if (ls >> v >> i >> j >> k >> l) d.integrals.emplace_back(v, i, j, k, l);
+  }
+  auto trim = [](std::string t) {
+    const size_t b = t.find_first_not_of(" \t");
+    if (b == std::string::npos) return std::string();
+    return t.substr(b, t.find_last_not_of(" \t") - b + 1);
+  };
+  std::stringstream hs(head);
+  std::string item;
+  while (std::getline(hs, item, ',')) {
+    const size_t eq = item.find('=');
+    if (eq == std::string::npos) continue;
+    d.header[trim(item.substr(0, eq))] = trim(item.substr(eq + 1));
   }
   return d;
 }
@@ -155,9 +159,22 @@ inline FCIDump LoadFCIDump(const std::string& filename) {
 template <typename ElemT>
 void SetupIntegrals(const FCIDump& fcidump, int& L, int& N, ElemT& I0, oneInt<ElemT>& I1,
                     twoInt<ElemT>& I2) {
+  bool have_norb = false;
   for (const auto& [key, value] : fcidump.header) {
-    if (key == "NORB") L = std::atoi(value.c_str());
+    if (key == "NORB") {
+      L = std::atoi(value.c_str());
+      have_norb = true;
+    }
     if (key == "NELEC") N = std::atoi(value.c_str());
+  }
+  // the reference goes on with whatever L held and writes out of bounds
+  if (!have_norb || L <= 0) throw std::invalid_argument("SetupIntegrals: FCIDUMP header has no positive NORB");
+  for (const auto& rec : fcidump.integrals) {
+    const int i = std::get<1>(rec), j = std::get<2>(rec), k = std::get<3>(rec), l = std::get<4>(rec);
+    auto in = [L](int p) { return p >= 1 && p <= L; };
+    const bool ok = (i == 0 && j == 0 && k == 0 && l == 0) || (k == 0 && l == 0 && in(i) && in(j)) ||
+                    (in(i) && in(j) && in(k) && in(l));
+    if (!ok) throw std::out_of_range("SetupIntegrals: integral record with an index outside 1..NORB");
   }
   I1.norbs = 2 * L;
   I1.store.assign(4 * L * L, ElemT(0.0));
@@ -549,7 +566,10 @@ void CarryOverAlphaDet(const std::vector<ElemT>& W, const std::vector<std::vecto
   std::vector<RealT> S(na);
   RealT sum = 0.0;
   for (size_t i = 0; i < na; ++i) S[i] = (sum += D[idx[i]]);
-  const size_t imax = static_cast<size_t>(std::upper_bound(S.begin(), S.end(), RealT(1.0) - threshold) - S.begin()) + 1;
+  // the smallest prefix whose weight exceeds 1 - threshold, plus one string; capped at every
+  // string (the reference reads past its table when the total weight never exceeds the target)
+  const size_t imax = std::min(
+      na, static_cast<size_t>(std::upper_bound(S.begin(), S.end(), RealT(1.0) - threshold) - S.begin()) + 1);
   rdet.resize(imax, std::vector<size_t>(adet[0].size()));
   for (size_t k = 0; k < imax; ++k) rdet[k] = adet[idx[k]];
 }
@@ -565,6 +585,7 @@ void CarryOverAlphaDet(const std::vector<ElemT>& W, const std::vector<std::vecto
   std::vector<size_t> idx(na);
   std::iota(idx.begin(), idx.end(), 0);
   std::sort(idx.begin(), idx.end(), [&](size_t i, size_t j) { return D[i] > D[j]; });
+  kept = std::min(kept, na);  // at most every string (the reference reads past its table)
   rdet.resize(kept, std::vector<size_t>(adet[0].size()));
   RealT sum = 0.0;
   for (size_t k = 0; k < kept; ++k) {

@@ -107,6 +107,8 @@ int main(int argc, char** argv) {
   int max_it = 100, max_nb = 10, reps = 1;
   double eps = 1e-8;
   size_t bit_length = 20;
+  std::vector<double> thresholds;
+  std::vector<size_t> kepts;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto next = [&]() { return std::string(argv[++i]); };
@@ -122,6 +124,8 @@ int main(int argc, char** argv) {
     else if (a == "--adets") apath = next();
     else if (a == "--bdets") bpath = next();
     else if (a == "--norb") norb = std::stoul(next());
+    else if (a == "--threshold") thresholds.push_back(std::stod(next()));
+    else if (a == "--kept") kepts.push_back(std::stoul(next()));
   }
   if (op == "files") {
     // determinant files through LoadAlphaDets (words written flat), and with --vec the
@@ -142,6 +146,40 @@ int main(int argc, char** argv) {
       sbd::SaveMatrixFormWF(outp + ".wf.txt", adet, bdet, 1, 1, h_comm, b_comm, t_comm, W);
     }
     std::printf("{\"op\":\"files\",\"na\":%zu,\"nb\":%zu}\n", adet.size(), bdet.size());
+    MPI_Finalize();
+    return 0;
+  }
+  if (op == "fcidump") {
+    // LoadFCIDump (framework/fcidump.h:27-95) on a text file, then SetupIntegrals
+    // (chemistry/basic/makeintegrals.h:10-55): the header as "key\tvalue" lines, the integral
+    // records, I0, I1 and I2
+    sbd::FCIDump d = sbd::LoadFCIDump(inpath);
+    {
+      std::ofstream h(outp + ".header.txt", std::ios::binary);
+      for (const auto& kv : d.header) h << kv.first << '\t' << kv.second << '\n';
+    }
+    std::vector<double> vals;
+    std::vector<int32_t> idx;
+    for (const auto& t : d.integrals) {
+      vals.push_back(std::get<0>(t));
+      idx.push_back(std::get<1>(t));
+      idx.push_back(std::get<2>(t));
+      idx.push_back(std::get<3>(t));
+      idx.push_back(std::get<4>(t));
+    }
+    write_vec(outp + ".ints.f64", vals);
+    write_vec(outp + ".ints.i32", idx);
+    int L = 0, N = 0;
+    double I0 = 0.0;
+    sbd::oneInt<double> I1;
+    sbd::twoInt<double> I2;
+    sbd::SetupIntegrals(d, L, N, I0, I1, I2);
+    std::vector<double> i0{I0};
+    write_vec(outp + ".I0.f64", i0);
+    write_vec(outp + ".I1.f64", I1.store);
+    write_vec(outp + ".I2.f64", I2.store);
+    std::printf("{\"op\":\"fcidump\",\"norb\":%d,\"nelec\":%d,\"nint\":%zu}\n", L, N,
+                d.integrals.size());
     MPI_Finalize();
     return 0;
   }
@@ -238,6 +276,53 @@ int main(int argc, char** argv) {
     std::printf("{\"op\":\"timemult\",\"ndet\":%zu,\"threads\":%d,\"reps\":%d,\"best_s\":%.6f,"
                 "\"mean_s\":%.6f,\"helper_s\":%.6f}\n",
                 ndet, omp_get_max_threads(), reps, best, tot / reps, t_help);
+  }
+  if (has("rdmvec")) {
+    // Correlation (chemistry/tpb/correlation.h:194-443) and OccupationDensity
+    // (chemistry/tpb/occupation.h:10-95) of a given vector
+    std::vector<double> W = read_vec(vecpath, ndet);
+    std::vector<std::vector<double>> one, two;
+    sbd::Correlation(W, adet, bdet, bit_length, static_cast<size_t>(L), 1, 1, helper, h_comm,
+                     b_comm, t_comm, one, two);
+    std::vector<double> o, t;
+    for (auto& v : one) o.insert(o.end(), v.begin(), v.end());
+    for (auto& v : two) t.insert(t.end(), v.begin(), v.end());
+    write_vec(outp + ".one.f64", o);
+    write_vec(outp + ".two.f64", t);
+    std::vector<int> oIdx(L);
+    std::iota(oIdx.begin(), oIdx.end(), 0);
+    std::vector<double> dens;
+    sbd::OccupationDensity(oIdx, W, adet, bdet, bit_length, 1, 1, b_comm, dens);
+    write_vec(outp + ".density.f64", dens);
+  }
+  if (has("carryover")) {
+    // CarryOverAlphaDet (chemistry/tpb/rdmat.h:60-132) of a given vector: the threshold form
+    // for every --threshold and the kept form for every --kept (selected alpha indices, in the
+    // reference's order, matched back to the sorted string table)
+    std::vector<double> W = read_vec(vecpath, ndet);
+    std::vector<int64_t> out;  // per case: count, then indices
+    auto index_of = [&](const std::vector<size_t>& w) {
+      for (size_t k = 0; k < adet.size(); ++k)
+        if (adet[k] == w) return static_cast<int64_t>(k);
+      return int64_t(-1);
+    };
+    std::vector<double> discarded;
+    for (double th : thresholds) {
+      std::vector<std::vector<size_t>> rdet;
+      sbd::CarryOverAlphaDet(W, adet, bdet, 1, 1, b_comm, rdet, th);
+      out.push_back(static_cast<int64_t>(rdet.size()));
+      for (auto& w : rdet) out.push_back(index_of(w));
+    }
+    for (size_t kept : kepts) {
+      std::vector<std::vector<size_t>> rdet;
+      double dw = 0.0;
+      sbd::CarryOverAlphaDet(W, adet, bdet, 1, 1, b_comm, kept, rdet, dw);
+      out.push_back(static_cast<int64_t>(rdet.size()));
+      for (auto& w : rdet) out.push_back(index_of(w));
+      discarded.push_back(dw);
+    }
+    write_vec(outp + ".carry.i64", out);
+    write_vec(outp + ".discarded.f64", discarded);
   }
   if (want("davidson")) {
     std::vector<double> W;

@@ -96,7 +96,8 @@ struct Prof {
 // Beta-single entries regrouped for the mixed (alpha single x beta single) pass: columns are
 // cut into tiles of consecutive beta strings holding at most kMixCap entries; inside a tile
 // entries are sorted by ket column jb.  meta = ib_local (bits 0-15) | pb = b*L + j (16-27) |
-// sign (bit 31).
+// sign (bit 31).  pos[k] is the position inside its tile of beta single k (the beta list's own
+// numbering), so column ib's entries are pos[off_s[ib] .. off_s[ib + 1]), ascending.
 struct MixTiles {
   int ntile = 0;
   int max_cols = 0;
@@ -104,6 +105,7 @@ struct MixTiles {
   DevBuf<int64_t> ent_off;   // ntile + 1
   DevBuf<int32_t> col0, ncol;
   DevBuf<uint32_t> jb, meta;
+  DevBuf<uint16_t> pos;      // nnz of the beta singles
 };
 
 struct Workspace {

@@ -19,8 +19,10 @@
 //   alpha double:                     stored value (sign and exchange included)
 //   alpha single x beta single:       s_a s_b (ai|bj)
 // Arithmetic: FP64 (FMA), or FP32 data whose products (exact in FP64) are accumulated in FP64
-// per output element and rounded to FP32 once, or (SBDX_ACC_TWOSUM) FP32 data accumulated in
-// FP32 with compensated partials (error-free product + TwoSum per term).
+// and rounded to FP32 once per pass (alpha rows, mixed, beta rows) when the pass adds its sum
+// to the output element, or (SBDX_ACC_TWOSUM) FP32 data accumulated in FP32 with compensated
+// partials (error-free product + TwoSum per term).  Every output element is summed in a fixed
+// order, so a build is bitwise reproducible.
 #include "common.h"
 
 #include <vector>
@@ -39,27 +41,37 @@ namespace {
 constexpr int kRowTile = 64;
 
 struct AccD {
+  using Wide = double;
   double s = 0.0;
   __device__ inline void add(double p) { s += p; }
   __device__ inline void fma(double a, double b) { s = __fma_rn(a, b, s); }
   __device__ inline double get() const { return s; }
+  __device__ inline Wide wide() const { return s; }
+  __device__ inline double apply(double y) const { return y + s; }
 };
 
 // FP32 data, FP64 accumulation: the product of two FP32 numbers is exact in FP64 (24 + 24
 // significand bits), so every term enters the sum unrounded and only the FP64 additions round;
 // one conversion per operand and one FP64 FMA per term, against ten FP32 operations for an
-// error-free FP32 product plus TwoSum compensation of similar accuracy
+// error-free FP32 product plus TwoSum compensation of similar accuracy.  Partial sums that are
+// combined (the mixed pass's per-entry sums) stay in FP64; each pass rounds to FP32 once, when
+// it adds its sum to the output element.
 struct AccX {
+  using Wide = double;
   double s = 0.0;
   __device__ inline void fma(float a, float b) {
     s = __fma_rn(static_cast<double>(a), static_cast<double>(b), s);
   }
+  __device__ inline void add(double p) { s += p; }
   __device__ inline float get() const { return static_cast<float>(s); }
+  __device__ inline Wide wide() const { return s; }
+  __device__ inline float apply(float y) const { return static_cast<float>(static_cast<double>(y) + s); }
 };
 
 // compensated FP32 (SBDX_ACC_TWOSUM): p = a*b exactly as p + e (FMA residual), s + p exactly
 // as t + err (TwoSum), errors gathered in c (Ogita-Rump-Oishi Dot2)
 struct AccF {
+  using Wide = float;
   float s = 0.0f, c = 0.0f;
   __device__ inline void add(float p) {
     const float t = __fadd_rn(s, p);
@@ -73,6 +85,11 @@ struct AccF {
     add(p);
   }
   __device__ inline float get() const { return s + c; }
+  __device__ inline Wide wide() const { return s + c; }
+  __device__ inline float apply(float y) {
+    add(y);
+    return get();
+  }
 };
 
 // accumulator of a precision and FP32 accumulation mode
@@ -200,9 +217,18 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
 // loaded once per workgroup; for every alpha single the integral row V[(a_k i_k), :] (sign
 // folded in) is staged in double-buffered LDS, and the ket row x[ja_k, :] is gathered along
 // sorted jb.  Workgroups are numbered so that all tiles of one row run on the same XCD and
-// share that row's ket rows in L2.  Per-entry sums are folded into the tile's columns with LDS
-// atomics once at the end; sigma is updated by its owner, with no global atomics.
+// share that row's ket rows in L2.  At the end every column sums its entries' partial sums in
+// ascending entry position (a fixed order: sigma is bitwise reproducible), and sigma is
+// updated by its owner, with no atomics.
 constexpr int kMixMaxCols = 1024;
+// Bytes of LDS the end-of-kernel fold stages per group of entries (the integral-row buffers
+// are reused for it); a group is kFoldP entries per thread.
+constexpr int kFoldBytes = 16384;
+__host__ __device__ constexpr int fold_group(int ept, int threads, int k, int wide) {
+  int p = ept;
+  while (p > 1 && p * threads * k * wide > kFoldBytes) p /= 2;
+  return p;
+}
 constexpr size_t kMixMaxLds = 160 * 1024;  // LDS of one CU
 // Variants of the kernel: entries per thread (kEPT), threads per workgroup (kMixThreads) and
 // alpha singles staged per barrier (kStage), coded as 1000000 (kStage - 1) + 100 * threads +
@@ -220,12 +246,12 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     int64_t na, int64_t nb, int L, const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket,
     const uint32_t* __restrict__ a_exc, int ntile, const int64_t* __restrict__ t_off,
     const int32_t* __restrict__ t_col0, const int32_t* __restrict__ t_ncol,
-    const uint32_t* __restrict__ e_jb, const uint32_t* __restrict__ e_meta, const T* __restrict__ V,
+    const uint32_t* __restrict__ e_jb, const uint32_t* __restrict__ e_meta,
+    const int64_t* __restrict__ b_off, const uint16_t* __restrict__ e_pos, const T* __restrict__ V,
     const T* __restrict__ x, T* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L2 = L * L;
   T* vs = reinterpret_cast<T*>(smem_raw);  // [2][kStage][L2]: double-buffered V rows
-  T* out = vs + 2 * kStage * L2;
 
   // XCD-aware numbering: workgroup g runs on XCD g % 8; XCD x takes rows ia = x (mod 8), all
   // tiles of a row back to back
@@ -252,7 +278,6 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     jb[j] = ok ? e_jb[e] : 0u;
     meta[j] = ok ? e_meta[e] : 0x0000ffffu;
   }
-  for (int c = tid; c < ncol * K; c += kMixThreads) out[c] = T(0);
 
   // V rows of the next stage's alpha singles: fetched into registers before the current
   // stage's gathers (their latency hides under them), written to LDS after them
@@ -312,21 +337,59 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     }
     if (more) commit(nxt, kk + kStage);
   }
-  __syncthreads();
+
+  // Fold.  Entry sums pass through LDS (over the integral-row buffers) kFoldP entries per
+  // thread at a time; entry position = tid + j * kMixThreads.  Column c of the tile owns the
+  // beta singles b_off[col0 + c] .. b_off[col0 + c + 1] - 1, whose positions e_pos lists in
+  // ascending order, and sums them in that order.
+  using Acc = typename AccOf<T, kAcc>::type;
+  using Wide = typename Acc::Wide;
+  constexpr int kFoldP = fold_group(kEPT, kMixThreads, K, static_cast<int>(sizeof(Wide)));
+  constexpr int kCols = kMixMaxCols / kMixThreads;
+  Wide* stage = reinterpret_cast<Wide*>(smem_raw);
+  int64_t cur[kCols], cend[kCols];
+  Acc col[kCols][K];
 #pragma unroll
-  for (int j = 0; j < kEPT; ++j) {
-    const uint32_t m = meta[j];
-    if ((m & 0xffffu) != 0xffffu) {
+  for (int u = 0; u < kCols; ++u) {
+    const int c = tid + u * kMixThreads;
+    cur[u] = c < ncol ? b_off[col0 + c] : 0;
+    cend[u] = c < ncol ? b_off[col0 + c + 1] : 0;
+  }
+#pragma unroll
+  for (int j0 = 0; j0 < kEPT; j0 += kFoldP) {
+    __syncthreads();  // the LDS is free: integral rows (first group) or the previous group
+#pragma unroll
+    for (int jj = 0; jj < kFoldP; ++jj) {
+      const bool neg = meta[j0 + jj] >> 31;
 #pragma unroll
       for (int v = 0; v < K; ++v) {
-        const T s = acc[j][v].get();
-        atomicAdd(&out[(m & 0xffffu) * K + v], (m >> 31) ? -s : s);
+        const Wide w = acc[j0 + jj][v].wide();
+        stage[(jj * kMixThreads + tid) * K + v] = neg ? -w : w;
+      }
+    }
+    __syncthreads();
+    const uint32_t lo = static_cast<uint32_t>(j0) * kMixThreads;
+    const uint32_t hi = lo + static_cast<uint32_t>(kFoldP) * kMixThreads;
+#pragma unroll
+    for (int u = 0; u < kCols; ++u) {
+      while (cur[u] < cend[u]) {
+        const uint32_t p = e_pos[cur[u]];
+        if (p >= hi) break;
+#pragma unroll
+        for (int v = 0; v < K; ++v) col[u][v].add(stage[(p - lo) * K + v]);
+        ++cur[u];
       }
     }
   }
-  __syncthreads();
   T* yr = y + (ia * nb + col0) * K;
-  for (int c = tid; c < ncol * K; c += kMixThreads) yr[c] += out[c];
+#pragma unroll
+  for (int u = 0; u < kCols; ++u) {
+    const int c = tid + u * kMixThreads;
+    if (c < ncol) {
+#pragma unroll
+      for (int v = 0; v < K; ++v) yr[c * K + v] = col[u][v].apply(yr[c * K + v]);
+    }
+  }
 }
 
 // out[c * R + r] = in[r * C + c] on K-value units; S x S tiles staged in LDS
@@ -441,7 +504,12 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   std::vector<uint32_t> e_jb, e_meta;
   e_jb.reserve(B.nnz_s);
   e_meta.reserve(B.nnz_s);
-  std::vector<std::pair<uint32_t, uint32_t>> buf;
+  std::vector<uint16_t> pos(B.nnz_s);
+  struct Ent {
+    uint32_t jb, meta;
+    int64_t k;
+  };
+  std::vector<Ent> buf;
   int64_t ib = 0;
   while (ib < nb) {
     const int64_t start = ib;
@@ -451,19 +519,23 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
       ++ib;
     }
     if (ib == start) throw Error(-1, "beta string with more singles than a mixed tile holds");
+    if (cap > 65536) throw Error(-1, "mixed tile larger than its 16-bit entry positions");
     buf.clear();
     for (int64_t b = start; b < ib; ++b)
       for (int64_t k = off[b]; k < off[b + 1]; ++k) {
         const uint32_t e = exc[k];
         const uint32_t pb = ((e >> 6) & 63) * static_cast<uint32_t>(L) + (e & 63);
         const uint32_t meta = static_cast<uint32_t>(b - start) | (pb << 16) | (((e >> 12) & 1u) << 31);
-        buf.emplace_back(ket[k], meta);
+        buf.push_back(Ent{ket[k], meta, k});
       }
-    std::stable_sort(buf.begin(), buf.end(), [](const auto& u, const auto& v) { return u.first < v.first; });
-    for (auto& p : buf) {
-      e_jb.push_back(p.first);
-      e_meta.push_back(p.second);
+    std::stable_sort(buf.begin(), buf.end(), [](const Ent& u, const Ent& v) { return u.jb < v.jb; });
+    for (size_t r = 0; r < buf.size(); ++r) {
+      e_jb.push_back(buf[r].jb);
+      e_meta.push_back(buf[r].meta);
+      pos[buf[r].k] = static_cast<uint16_t>(r);
     }
+    // each column's positions ascending: the order in which the kernel sums them
+    for (int64_t b = start; b < ib; ++b) std::sort(pos.begin() + off[b], pos.begin() + off[b + 1]);
     t_col0.push_back(static_cast<int32_t>(start));
     t_ncol.push_back(static_cast<int32_t>(ib - start));
     t_off.push_back(static_cast<int64_t>(e_jb.size()));
@@ -475,11 +547,13 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   mt.ncol.alloc(t_ncol.size());
   mt.jb.alloc(e_jb.size());
   mt.meta.alloc(e_meta.size());
+  mt.pos.alloc(pos.size());
   SBDX_HIP(hipMemcpy(mt.ent_off.get(), t_off.data(), t_off.size() * 8, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.col0.get(), t_col0.data(), t_col0.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.ncol.get(), t_ncol.data(), t_ncol.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.jb.get(), e_jb.data(), e_jb.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.meta.get(), e_meta.data(), e_meta.size() * 4, hipMemcpyHostToDevice));
+  SBDX_HIP(hipMemcpy(mt.pos.get(), pos.data(), pos.size() * 2, hipMemcpyHostToDevice));
   return mt;
 }
 
@@ -499,22 +573,27 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
   if (mt.ntile == 0) return;
   const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
-  const size_t lds = (2 * static_cast<size_t>(stages) * c.L * c.L + static_cast<size_t>(mt.max_cols) * K) * sizeof(T);
-  if (lds > kMixMaxLds) throw Error(SBDX_ERR_ARG, "mixed-kernel variant needs more LDS than a CU has for this problem");
-  auto launch = [&](auto kernel, int threads) {
+  using Wide = typename AccOf<T, kAcc>::type::Wide;
+  auto launch = [&](auto kernel, int threads, int ept) {
+    // LDS: the double-buffered integral rows, reused at the end by the fold's entry groups
+    const size_t vrows = 2 * static_cast<size_t>(stages) * c.L * c.L * sizeof(T);
+    const size_t fold = static_cast<size_t>(fold_group(ept, threads, K, static_cast<int>(sizeof(Wide)))) *
+                        threads * K * sizeof(Wide);
+    const size_t lds = std::max(vrows, fold);
+    if (lds > kMixMaxLds) throw Error(SBDX_ERR_ARG, "mixed-kernel variant needs more LDS than a CU has for this problem");
     if (lds > 65536)
       SBDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     kernel<<<static_cast<unsigned>(blocks), threads, lds, st>>>(
         c.na, c.nb, c.L, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
-        mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), V, x, y);
+        mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), c.B.off_s.get(), mt.pos.get(), V, x, y);
     SBDX_CHECK_LAUNCH();
   };
   const bool small_v = static_cast<int>(c.L) * c.L <= 2048;
   auto pick = [&](auto e, auto t, auto sg) {
     constexpr int E = decltype(e)::value, NT = decltype(t)::value, SG = decltype(sg)::value;
-    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc>, NT)
-            : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc>, NT);
+    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc>, NT, E)
+            : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc>, NT, E);
   };
   using std::integral_constant;
   using I4 = integral_constant<int, 4>;
@@ -669,11 +748,13 @@ std::vector<int> mix_variant_candidates(int K) {
   return {25604, 51204};
 }
 
-// whether a variant's LDS (double-buffered V rows of its stages + the tile's columns, at most
-// kMixMaxCols) fits a CU for this problem
+// whether a variant's LDS (double-buffered V rows of its stages, reused by the fold's entry
+// groups of FP64 partial sums) fits a CU for this problem
 bool mix_variant_fits(const sbdx_ctx& c, int prec, int K, int var) {
   const size_t stages = var / 1000000 + 1, es = prec == 0 ? 8 : 4;
-  return (2 * stages * c.L * c.L + static_cast<size_t>(kMixMaxCols) * K) * es <= kMixMaxLds;
+  const int threads = (var % 1000000) / 100, ept = var % 100;
+  const size_t fold = static_cast<size_t>(fold_group(ept, threads, K, 8)) * threads * K * 8;
+  return std::max(2 * stages * c.L * c.L * es, fold) <= kMixMaxLds;
 }
 
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {

@@ -1,0 +1,135 @@
+// FCIDUMP reading and carry-over selection through the drop-in C++ API
+// (include/sbd_amd/sbd.h).  Host code only: no GPU call.  tests/test_shim_post.py compares the
+// files it writes with the compiled reference's (tests/golden/io_post.npz).
+//
+//   shim_post fcidump <FCIDUMP file> <out prefix>
+//       LoadFCIDump + SetupIntegrals: <out>.header.txt ("key\tvalue" lines), <out>.ints.f64,
+//       <out>.ints.i32 (i j k l per record), <out>.I0.f64, <out>.I1.f64, <out>.I2.f64; on a
+//       std::exception prints {"error": ...} and exits 4
+//   shim_post carry <alpha u64> <beta u64> <norb> <W f64> <out prefix> [--threshold t]... [--kept k]...
+//       CarryOverAlphaDet in both forms: <out>.carry.i64 (per case: count, then the indices of
+//       the selected strings in the alpha table), <out>.discarded.f64 (kept form)
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "sbd_amd/sbd.h"
+
+namespace {
+
+template <class T>
+void write_vec(const std::string& path, const std::vector<T>& v) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(sizeof(T) * v.size()));
+}
+
+template <class T>
+std::vector<T> read_all(const std::string& path) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  const std::streamsize n = f.tellg();
+  std::vector<T> v(static_cast<size_t>(n) / sizeof(T));
+  f.seekg(0);
+  f.read(reinterpret_cast<char*>(v.data()), n);
+  return v;
+}
+
+// uint64 masks (bit p = orbital p) -> words of 20 bits, least significant first
+std::vector<std::vector<size_t>> to_words(const std::vector<uint64_t>& s, size_t norb) {
+  const size_t bl = 20, nw = (norb + bl - 1) / bl;
+  std::vector<std::vector<size_t>> out(s.size(), std::vector<size_t>(nw, 0));
+  for (size_t k = 0; k < s.size(); ++k)
+    for (size_t p = 0; p < norb; ++p)
+      if ((s[k] >> p) & 1u) out[k][p / bl] |= size_t(1) << (p % bl);
+  return out;
+}
+
+int run_fcidump(const std::string& file, const std::string& out) {
+  sbd::FCIDump d = sbd::LoadFCIDump(file);
+  {
+    std::ofstream h(out + ".header.txt", std::ios::binary);
+    for (const auto& kv : d.header) h << kv.first << '\t' << kv.second << '\n';
+  }
+  std::vector<double> vals;
+  std::vector<int32_t> idx;
+  for (const auto& t : d.integrals) {
+    vals.push_back(std::get<0>(t));
+    for (int v : {std::get<1>(t), std::get<2>(t), std::get<3>(t), std::get<4>(t)}) idx.push_back(v);
+  }
+  write_vec(out + ".ints.f64", vals);
+  write_vec(out + ".ints.i32", idx);
+  int L = 0, N = 0;
+  double I0 = 0.0;
+  sbd::oneInt<double> I1;
+  sbd::twoInt<double> I2;
+  sbd::SetupIntegrals(d, L, N, I0, I1, I2);
+  write_vec(out + ".I0.f64", std::vector<double>{I0});
+  write_vec(out + ".I1.f64", I1.store);
+  write_vec(out + ".I2.f64", I2.store);
+  std::printf("{\"op\":\"fcidump\",\"norb\":%d,\"nelec\":%d,\"nint\":%zu}\n", L, N, d.integrals.size());
+  return 0;
+}
+
+int run_carry(int argc, char** argv) {
+  const auto alpha = read_all<uint64_t>(argv[2]);
+  const auto beta = read_all<uint64_t>(argv[3]);
+  const size_t norb = std::stoul(argv[4]);
+  const auto W = read_all<double>(argv[5]);
+  const std::string out = argv[6];
+  std::vector<double> thresholds;
+  std::vector<size_t> kepts;
+  for (int i = 7; i + 1 < argc; i += 2) {
+    const std::string a = argv[i];
+    if (a == "--threshold") thresholds.push_back(std::stod(argv[i + 1]));
+    else if (a == "--kept") kepts.push_back(std::stoul(argv[i + 1]));
+  }
+  const auto adet = to_words(alpha, norb), bdet = to_words(beta, norb);
+  auto index_of = [&](const std::vector<size_t>& w) {
+    for (size_t k = 0; k < adet.size(); ++k)
+      if (adet[k] == w) return static_cast<int64_t>(k);
+    return int64_t(-1);
+  };
+  MPI_Comm h_comm, b_comm, t_comm;
+  sbd::TaskCommunicator(MPI_COMM_WORLD, 1, 1, 1, 1, h_comm, b_comm, t_comm);
+  std::vector<int64_t> res;
+  std::vector<double> discarded;
+  for (double th : thresholds) {
+    std::vector<std::vector<size_t>> rdet;
+    sbd::CarryOverAlphaDet(W, adet, bdet, 1, 1, b_comm, rdet, th);
+    res.push_back(static_cast<int64_t>(rdet.size()));
+    for (const auto& w : rdet) res.push_back(index_of(w));
+  }
+  for (size_t kept : kepts) {
+    std::vector<std::vector<size_t>> rdet;
+    double dw = 0.0;
+    sbd::CarryOverAlphaDet(W, adet, bdet, 1, 1, b_comm, kept, rdet, dw);
+    res.push_back(static_cast<int64_t>(rdet.size()));
+    for (const auto& w : rdet) res.push_back(index_of(w));
+    discarded.push_back(dw);
+  }
+  write_vec(out + ".carry.i64", res);
+  write_vec(out + ".discarded.f64", discarded);
+  std::printf("{\"op\":\"carry\",\"cases\":%zu}\n", thresholds.size() + kepts.size());
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  int provided;
+  MPI_Init_thread(&argc, &argv, MPI_THREAD_FUNNELED, &provided);
+  int rc = 2;
+  const std::string op = argc > 1 ? argv[1] : "";
+  try {
+    if (op == "fcidump" && argc == 4) rc = run_fcidump(argv[2], argv[3]);
+    else if (op == "carry" && argc >= 7) rc = run_carry(argc, argv);
+    else std::fprintf(stderr, "usage: shim_post fcidump <file> <out> | carry <alpha> <beta> <norb> <W> <out> ...\n");
+  } catch (const std::exception& e) {
+    std::printf("{\"error\":\"%s\"}\n", e.what());
+    rc = 4;
+  }
+  MPI_Finalize();
+  return rc;
+}

@@ -1,7 +1,8 @@
 """The HIP engine against the compiled reference's golden fixtures (tests/golden/).
 
 Lists bit-exact; H_ii and sigma to 1e-12 relative; Davidson energies to 1e-9 Ha; RDMs of the
-converged vector to 1e-6 (both solves stop at a 1e-8 residual).
+converged vector to 1e-6 (both solves stop at a 1e-8 residual); RDMs of a seeded vector to
+1e-12 (36 and 48 orbitals).
 """
 import json
 import os
@@ -56,7 +57,7 @@ def test_golden_lists_hdiag_sigma(name):
         compare(z, f"sigma{s}", eng.sigma(x), 1e-12)
 
 
-@pytest.mark.parametrize("name", ["l6_2a2b", "l8_3a2b", "c2fam_120x110"])
+@pytest.mark.parametrize("name", CASES)
 def test_golden_davidson_and_rdm(name):
     print(f"[golden davidson {name}]", flush=True)
     meta, z = load(name)
@@ -68,6 +69,29 @@ def test_golden_davidson_and_rdm(name):
     assert np.max(np.abs(one.ravel() - z["one"])) < 1e-6
     assert np.max(np.abs(two.ravel()[z["two_idx"]] - z["two_val"])) < 1e-6
     assert np.max(np.abs(eng.occupation(r["W"]) - z["density"])) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["c2fam_300x280", "c3fam_200x220"])
+def test_golden_rdm_seeded_vector(name):
+    """Correlation and OccupationDensity of one normalised seeded vector: every stored 2-RDM
+    entry (the largest 150,000 or all), the whole of the 1-RDM and density, and whole-array
+    checksums of the 2-RDM (a cosine-weighted sum moves with any single entry)."""
+    print(f"[golden rdm {name}]", flush=True)
+    meta, z = load(name)
+    inst, eng = build(meta)
+    W = np.random.default_rng(meta["rdmvec_seed"]).standard_normal(eng.ndet)
+    W /= np.linalg.norm(W)
+    one, two = eng.rdm(W)
+    one, two = one.ravel(), two.ravel()
+    assert np.max(np.abs(one - z["rdmvec_one"])) < 1e-12
+    assert np.max(np.abs(two[z["rdmvec_two_idx"]] - z["rdmvec_two_val"])) < 1e-12
+    assert two.size == int(z["rdmvec_two_size"][0])
+    scale = float(np.sum(np.abs(two)))
+    assert abs(two.sum() - float(z["rdmvec_two_sum"][0])) < 1e-12 * scale
+    assert abs(two @ two - float(z["rdmvec_two_sumsq"][0])) < 1e-12 * max(1.0, float(z["rdmvec_two_sumsq"][0]))
+    w = two @ np.cos(0.1 * np.arange(two.size))
+    assert abs(w - float(z["rdmvec_two_wsum"][0])) < 1e-12 * scale
+    assert np.max(np.abs(eng.occupation(W) - z["rdmvec_density"])) < 1e-12
 
 
 def test_golden_c1_davidson_rdm_energy_identity():

@@ -113,9 +113,8 @@ def test_record_matches_oracle():
 
 def test_zero_last_layer_is_the_diagonal_preconditioner():
     """A network with w3 = b3 = 0 outputs m = 0: preconditioner 1 returns r / (theta - H_ii)
-    to the last bit or two, and the solver follows the diagonal preconditioner's iteration (to the
-    rounding of the FP32 sigma, whose mixed pass folds entries with LDS atomics in no fixed
-    order) to the same energy."""
+    to the last bit or two, and the solver follows the diagonal preconditioner's iteration (to
+    the rounding of those last bits, which the iteration amplifies) to the same energy."""
     inst, so, eng = small_case()
     hii = eng.hdiag()  # the engine's own diagonal (the oracle's agrees to 1e-12, not bitwise)
     r = np.random.default_rng(8).standard_normal(so.ndet)

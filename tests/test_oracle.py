@@ -89,7 +89,8 @@ def test_sigma_bit_exact(name):
         check_vector(z, f"sigma{s}", so.mult(x, hii), exact=True)
 
 
-@pytest.mark.parametrize("name", ["l6_2a2b", "l8_3a2b", "c2fam_120x110"])
+# c2fam_300x280's Davidson takes ~25 s here; its energy is pinned on the GPU side
+@pytest.mark.parametrize("name", ["l6_2a2b", "l8_3a2b", "c2fam_120x110", "c3fam_200x220"])
 def test_davidson_energy_and_rdm(name):
     meta, z, inst, so = system(name)
     r = so.davidson(max_it=100, num_block=10, eps=1e-8)
@@ -105,6 +106,24 @@ def test_davidson_energy_and_rdm(name):
     assert np.max(np.abs(two.ravel()[mask] - ref_two[mask])) < 1e-6
     assert abs(two.sum() - float(z["two_sum"][0])) < 1e-6 * max(1.0, abs(float(z["two_sum"][0])))
     assert np.max(np.abs(so.occupation(r["W"]) - z["density"])) < 1e-6
+
+
+@pytest.mark.parametrize("name", ["c2fam_300x280", "c3fam_200x220"])
+def test_rdm_of_seeded_vector_matches_reference(name):
+    """Correlation and OccupationDensity of the same seeded vector as the compiled reference:
+    36 orbitals (two 20-bit words per string) and 48 (three)."""
+    meta, z, inst, so = system(name)
+    W = np.random.default_rng(meta["rdmvec_seed"]).standard_normal(so.ndet)
+    W /= np.linalg.norm(W)
+    one, two = so.correlation(W)
+    one, two = one.ravel(), two.ravel()
+    assert np.max(np.abs(one - z["rdmvec_one"])) < 1e-15
+    assert np.max(np.abs(two[z["rdmvec_two_idx"]] - z["rdmvec_two_val"])) < 1e-15
+    assert int(np.count_nonzero(np.abs(two) > 1e-14)) == int(z["rdmvec_two_nnz"][0])
+    scale = float(np.sum(np.abs(two)))
+    assert abs(two @ np.cos(0.1 * np.arange(two.size)) - float(z["rdmvec_two_wsum"][0])) < 1e-14 * scale
+    # the reference's occupation sums run over OpenMP threads: same terms, another order
+    assert np.max(np.abs(so.occupation(W) - z["rdmvec_density"])) < 1e-12
 
 
 def test_c1_fixture_consistency():

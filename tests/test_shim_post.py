@@ -1,0 +1,144 @@
+"""FCIDUMP reading and carry-over selection through the drop-in C++ API (include/sbd_amd/sbd.h)
+against the compiled reference's outputs for the same inputs (tests/golden/io_post.npz, written
+by tests/golden/make_golden.py from oracle/_ref/sbd_ref).  Host code only.
+
+LoadFCIDump + SetupIntegrals (framework/fcidump.h:27-95, chemistry/basic/makeintegrals.h:10-55)
+on the committed FCIDUMP texts: header map, integral records, I0, I1 and I2 identical.
+CarryOverAlphaDet (chemistry/tpb/rdmat.h:60-132), threshold and kept forms, on a vector whose
+alpha rows tie exactly: the same strings in the same order.  Where the reference reads or
+writes out of bounds (a lost NORB; a threshold or count that selects past the table) the
+drop-in raises or caps, and those cases are checked here against what the mathematics fixes.
+"""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CPP = os.path.join(ROOT, "tests", "cpp")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+@pytest.fixture(scope="module")
+def shim_post():
+    if not os.path.exists("/opt/conda/include/mpi.h") or shutil.which("g++") is None:
+        pytest.skip("needs g++ and the image's MPI to build tests/cpp/shim_post")
+    r = subprocess.run(["make", "-s", "-C", CPP, "shim_post"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    return os.path.join(CPP, "shim_post")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    z = np.load(os.path.join(GOLDEN, "io_post.npz"), allow_pickle=False)
+    return json.loads(str(z["meta"])), z
+
+
+def run(args):
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    line = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    return r.returncode, json.loads(line[-1]) if line else None
+
+
+@pytest.mark.parametrize("name", ["std", "slash", "joined"])
+def test_fcidump_matches_reference(shim_post, golden, tmp_path, name):
+    meta, z = golden
+    out = str(tmp_path / name)
+    rc, line = run([shim_post, "fcidump", os.path.join(GOLDEN, "fcidump", f"{name}.FCIDUMP"), out])
+    assert rc == 0, line
+    with open(out + ".header.txt") as f:
+        assert f.read() == str(z[f"fcidump_{name}_header"])
+    vals = np.fromfile(out + ".ints.f64", np.float64)
+    idx = np.fromfile(out + ".ints.i32", np.int32)
+    assert np.array_equal(vals, z[f"fcidump_{name}_vals"])
+    assert np.array_equal(idx, z[f"fcidump_{name}_idx"])
+    for t in ("I0", "I1", "I2"):
+        assert np.array_equal(np.fromfile(f"{out}.{t}.f64", np.float64), z[f"fcidump_{name}_{t}"]), t
+
+
+def test_fcidump_layout_by_hand(golden):
+    """What the fixtures say, restated: ORBSYM keeps its first value, a '/' terminator leaves
+    every line in the header (no integrals), a key whose line has no trailing comma swallows
+    the next line's text."""
+    _, z = golden
+    std = dict(ln.split("\t") for ln in str(z["fcidump_std_header"]).splitlines())
+    assert std == {"NORB": "4", "NELEC": "4", "MS2": "0", "ORBSYM": "1", "ISYM": "1"}
+    assert z["fcidump_std_vals"].size == 55 + 10 + 1
+    assert z["fcidump_slash_vals"].size == 0 and float(z["fcidump_slash_I0"][0]) == 0.0
+    joined = dict(ln.split("\t") for ln in str(z["fcidump_joined_header"]).splitlines())
+    assert joined["MS2"] == "0 ISYM=1" and "ISYM" not in joined
+
+
+def test_fcidump_without_norb_raises(shim_post, tmp_path):
+    # all of the header on the &END line: the reader drops that line, so NORB is lost
+    path = tmp_path / "oneline.FCIDUMP"
+    path.write_text("&FCI NORB=2,NELEC=2,MS2=0 &END\n 0.5 1 1 1 1\n -1.0 1 1 0 0\n")
+    rc, line = run([shim_post, "fcidump", str(path), str(tmp_path / "o")])
+    assert rc == 4 and "NORB" in line["error"]
+    path.write_text("&FCI NORB=2,NELEC=2,\n&END\n 0.5 3 1 1 1\n")
+    rc, line = run([shim_post, "fcidump", str(path), str(tmp_path / "o")])
+    assert rc == 4 and "index" in line["error"]
+
+
+def carry_cases(out, thresholds, kept):
+    res = np.fromfile(out + ".carry.i64", np.int64)
+    cases, p = [], 0
+    for _ in range(len(thresholds) + len(kept)):
+        n = int(res[p])
+        cases.append(res[p + 1:p + 1 + n])
+        p += 1 + n
+    assert p == res.size
+    return cases
+
+
+def carry_run(shim_post, tmp_path, alpha, beta, W, thresholds, kept):
+    d = tmp_path
+    alpha.astype(np.uint64).tofile(d / "a.u64")
+    beta.astype(np.uint64).tofile(d / "b.u64")
+    W.astype(np.float64).tofile(d / "w.f64")
+    args = [shim_post, "carry", str(d / "a.u64"), str(d / "b.u64"), "8", str(d / "w.f64"), str(d / "c")]
+    for t in thresholds:
+        args += ["--threshold", repr(float(t))]
+    for k in kept:
+        args += ["--kept", str(int(k))]
+    rc, line = run(args)
+    assert rc == 0, line
+    return carry_cases(str(d / "c"), thresholds, kept), np.fromfile(str(d / "c") + ".discarded.f64")
+
+
+def test_carryover_matches_reference(shim_post, golden, tmp_path):
+    _, z = golden
+    th, kept = list(z["carry_thresholds"]), list(z["carry_kept"])
+    got, disc = carry_run(shim_post, tmp_path, z["carry_alpha"], z["carry_beta"], z["carry_W"], th, kept)
+    res, p = z["carry_out"], 0
+    for case in got:
+        n = int(res[p])
+        assert np.array_equal(case, res[p + 1:p + 1 + n])
+        p += 1 + n
+    assert np.array_equal(disc, z["carry_discarded"])
+    # the selection rule by hand: weights descending, prefix passing 1 - threshold, plus one
+    na, nb = z["carry_alpha"].size, z["carry_beta"].size
+    D = (z["carry_W"].reshape(na, nb) ** 2).sum(1)
+    for t, case in zip(th, got):
+        assert np.all(np.diff(D[case]) <= 0)
+        S = np.cumsum(D[case])
+        assert S[-2] <= 1 - t < S[-1] if case.size > 1 else S[-1] > 1 - t
+    # the tied rows (3, 5, 17, 40 share one weight) stay together
+    full = got[len(th) + len(kept) - 1]
+    pos = sorted(int(np.flatnonzero(full == r)[0]) for r in (3, 5, 17, 40))
+    assert pos == list(range(pos[0], pos[0] + 4))
+
+
+def test_carryover_caps(shim_post, golden, tmp_path):
+    """Threshold 0 with a total weight just under 1, and more strings kept than exist: every
+    string, once (the reference selects past its table in both cases)."""
+    _, z = golden
+    na = z["carry_alpha"].size
+    W = z["carry_W"] * (1 - 1e-9)
+    got, disc = carry_run(shim_post, tmp_path, z["carry_alpha"], z["carry_beta"], W, [0.0], [na + 7])
+    for case in got:
+        assert case.size == na and sorted(case) == list(range(na))
+    assert abs(disc[0] - (1 - (1 - 1e-9) ** 2)) < 1e-12
