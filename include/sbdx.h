@@ -161,6 +161,10 @@ int sbdx_davidson_device(sbdx_ctx *ctx, const sbdx_davidson_opts *opts, void *dW
  * (4 L^4) = <c+_{is} c+_{jt} c_{lt} c_{ks}> (the reference's layout, user-guide.md:136-139). */
 int sbdx_rdm(sbdx_ctx *ctx, const double *W, double *onebody, double *twobody);
 
+/* Device-resident variant: dW is a device buffer of ndet doubles (the interleaved basis, as
+ * sbdx_davidson_device leaves it); the RDMs are written to host memory. */
+int sbdx_rdm_device(sbdx_ctx *ctx, const void *dW, double *onebody, double *twobody);
+
 /* Occupations density[2p + s] = <n_{p,s}> of W (host). */
 int sbdx_occupation(sbdx_ctx *ctx, const double *W, double *density);
 
@@ -268,6 +272,15 @@ int sbdx_set_mix_variant(sbdx_ctx *ctx, int prec, int k, int variant);
  * Enabling resets the sums. */
 int sbdx_profile(sbdx_ctx *ctx, int enable);
 int sbdx_profile_read(sbdx_ctx *ctx, double *ms, int cap, int64_t *calls);
+
+/* Measurement hook, no counterpart in the reference: bandwidth ceiling of a memory level of the
+ * current device, in GB/s, over `reps` timed launches.  SBDX_PROBE_HBM: STREAM-style copy of
+ * `bytes` (2 x bytes moved per launch; use a buffer far above the L2s and the MALL).
+ * SBDX_PROBE_L2: every workgroup reads the whole of an L2-resident buffer of `bytes` (at most
+ * 2 MiB) in 16-byte loads (workgroups x bytes moved per launch, L2 to L1).  Allocates. */
+#define SBDX_PROBE_HBM 0
+#define SBDX_PROBE_L2 1
+int sbdx_bandwidth_probe(int kind, int64_t bytes, int reps, double *gbs);
 
 void sbdx_destroy(sbdx_ctx *ctx);
 

@@ -24,6 +24,7 @@ EXPORTS = [
     "sbdx_mlp_train_defaults", "sbdx_mlp_record", "sbdx_mlp_loss_grad", "sbdx_mlp_forward",
     "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune", "sbdx_set_mix_variant", "sbdx_set_fp32_accumulation",
     "sbdx_batch_create", "sbdx_batch_davidson", "sbdx_batch_destroy", "sbdx_store_hamiltonian",
+    "sbdx_bandwidth_probe", "sbdx_rdm_device",
 ]
 MLP_NPARAM = 12609
 MLP_SAMPLE = np.dtype([("alpha", "<u8"), ("beta", "<u8"), ("xh", "<f4"), ("xr", "<f4"),
@@ -89,6 +90,7 @@ def lib():
     L.sbdx_davidson_device.argtypes = [vp, C.POINTER(DavidsonOpts), vp, C.POINTER(DavidsonResult),
                                        vp, vp, C.c_int]
     L.sbdx_rdm.argtypes = [vp, f64p, f64p, f64p]
+    L.sbdx_rdm_device.argtypes = [vp, vp, f64p, f64p]
     L.sbdx_occupation.argtypes = [vp, f64p, f64p]
     L.sbdx_profile.argtypes = [vp, C.c_int]
     L.sbdx_profile_read.argtypes = [vp, f64p, C.c_int, C.POINTER(C.c_int64)]
@@ -117,6 +119,7 @@ def lib():
     L.sbdx_batch_destroy.argtypes = [vp]
     L.sbdx_store_hamiltonian.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.sbdx_batch_destroy.restype = None
+    L.sbdx_bandwidth_probe.argtypes = [C.c_int, C.c_int64, C.c_int, C.POINTER(C.c_double)]
     for name in EXPORTS:
         if name not in ("sbdx_ndet", "sbdx_davidson_defaults", "sbdx_destroy", "sbdx_last_error",
                         "sbdx_mlp_train_defaults", "sbdx_batch_destroy"):
@@ -177,6 +180,14 @@ def mlp_train(samples, params, **opts):
     trace = np.zeros(max(1, o.epochs))
     check(lib().sbdx_mlp_train(s, s.size, C.byref(o), p, trace.ctypes.data, trace.size))
     return p, trace[:o.epochs].copy()
+
+
+def bandwidth_probe(kind, nbytes, reps):
+    """GB/s of a memory level of the current device (sbdx_bandwidth_probe): kind 0 HBM copy,
+    kind 1 L2 -> L1 read of an L2-resident buffer."""
+    g = C.c_double()
+    check(lib().sbdx_bandwidth_probe(int(kind), int(nbytes), int(reps), C.byref(g)))
+    return g.value
 
 
 class Engine:
@@ -342,6 +353,14 @@ class Engine:
         one = np.zeros(2 * L * L)
         two = np.zeros(4 * L ** 4)
         check(lib().sbdx_rdm(self._h, np.ascontiguousarray(W, np.float64), one, two))
+        return one.reshape(2, L * L), two.reshape(4, L ** 4)
+
+    def rdm_device(self, dW):
+        """RDMs of a device vector (sbdx_rdm_device): dW is a device pointer to ndet doubles."""
+        L = self.norb
+        one = np.zeros(2 * L * L)
+        two = np.zeros(4 * L ** 4)
+        check(lib().sbdx_rdm_device(self._h, C.c_void_p(dW), one, two))
         return one.reshape(2, L * L), two.reshape(4, L ** 4)
 
     def occupation(self, W):

@@ -27,6 +27,7 @@ void mlp_loss_grad(const sbdx_mlp_sample* S, int64_t n, const float* params, dou
 void mlp_forward(const sbdx_mlp_sample* S, int64_t n, const float* params, float* m);
 void mlp_train(const sbdx_mlp_sample* S, int64_t n, const sbdx_mlp_train_opts& o, float* params, double* trace,
                int trace_cap);
+double bandwidth_probe(int kind, int64_t bytes, int reps);
 }  // namespace sbdx
 
 namespace {
@@ -308,6 +309,16 @@ int sbdx_rdm(sbdx_ctx* c, const double* W, double* onebody, double* twobody) {
   });
 }
 
+int sbdx_rdm_device(sbdx_ctx* c, const void* dW, double* onebody, double* twobody) {
+  return guarded([&] {
+    need_ints(c);
+    need(dW && onebody && twobody, "null argument");
+    sbdx::DevBuf<double> w(c->ndet());
+    sbdx::apply_sign(*c, SBDX_FP64, 1, dW, w.get(), c->stream);
+    sbdx::rdm_blocked(*c, w.get(), onebody, twobody);
+  });
+}
+
 int sbdx_occupation(sbdx_ctx* c, const double* W, double* density) {
   return guarded([&] {
     need(c && W && density, "null argument");
@@ -534,6 +545,13 @@ int sbdx_profile_read(sbdx_ctx* c, double* ms, int cap, int64_t* calls) {
     need(c && ms, "null argument");
     for (int k = 0; k < cap && k < sbdx::kPhases; ++k) ms[k] = c->prof.ms[k];
     if (calls) *calls = c->prof.calls;
+  });
+}
+
+int sbdx_bandwidth_probe(int kind, int64_t bytes, int reps, double* gbs) {
+  return guarded([&] {
+    need(gbs != nullptr, "null argument");
+    *gbs = sbdx::bandwidth_probe(kind, bytes, reps);
   });
 }
 

@@ -94,10 +94,10 @@ struct Prof {
 };
 
 // Beta-single entries regrouped for the mixed (alpha single x beta single) pass: columns are
-// cut into tiles of consecutive beta strings holding at most kMixCap entries; inside a tile
-// entries are sorted by ket column jb.  meta = ib_local (bits 0-15) | pb = b*L + j (16-27) |
-// sign (bit 31).  pos[k] is the position inside its tile of beta single k (the beta list's own
-// numbering), so column ib's entries are pos[off_s[ib] .. off_s[ib + 1]), ascending.
+// cut into tiles of consecutive beta strings holding at most cap entries; inside a tile
+// entries are sorted by ket column jb.  meta = slot (bits 0-15: the entry's index in the beta
+// list minus the tile's first index, so each column's entries have consecutive slots in the
+// list's own order) | pb = b*L + j (16-27) | sign (bit 31).
 struct MixTiles {
   int ntile = 0;
   int max_cols = 0;
@@ -105,7 +105,6 @@ struct MixTiles {
   DevBuf<int64_t> ent_off;   // ntile + 1
   DevBuf<int32_t> col0, ncol;
   DevBuf<uint32_t> jb, meta;
-  DevBuf<uint16_t> pos;      // nnz of the beta singles
 };
 
 struct Workspace {
@@ -132,6 +131,11 @@ struct sbdx_ctx {
   sbdx::DevBuf<float> eri4_32;
   sbdx::DevBuf<double> hii;        // H_ii, blocked and interleaved bases agree
   sbdx::DevBuf<float> hii32;
+  // Coulomb tables of the same-spin singles' spectator term: coul_a[p * nb + ib] =
+  // sum_{j in beta(ib)} (p|jj) for every orbital pair p = a L + i (alpha-row pass), coul_b
+  // [p * na + ia] = sum_{j in alpha(ia)} (p|jj) (beta-row pass); j ascending
+  sbdx::DevBuf<double> coul_a, coul_b;
+  sbdx::DevBuf<float> coul_a32, coul_b32;
   sbdx::Workspace ws;
   sbdx::Prof prof;
   int mix_variant_tuned[2][4] = {}; // autotuned mixed-kernel variant [FP64|FP32][K slot]
@@ -226,6 +230,7 @@ void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta);  //
 void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
 void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
 void reset_mix_tiles(sbdx_ctx& c);              // sigma.hip: drop cached mixed-pass tiles
+void prepare_mix_tiles(sbdx_ctx& c, int prec, int k);  // sigma.hip: build the default variant's tiles
 std::vector<int> mix_variant_candidates(int k); // sigma.hip: autotuner search space
 bool mix_variant_fits(const sbdx_ctx& c, int prec, int k, int variant);  // its LDS fits a CU
 // sigma of nvec vectors stored one after another (vector-major)

@@ -12,7 +12,9 @@
 //                    orbital pairs pa = (a, i), pb = (b, j)
 // and the host expands them into the reference's index patterns (ZeroDiffCorrelation
 // :15-44, OneDiffCorrelation :49-86, TwoDiffCorrelation :91-130).  In the blocked basis the
-// interleaved signs reduce to one-spin signs (see common.h).
+// interleaved signs reduce to one-spin signs (see common.h).  No global atomics: row and pair
+// sums are wavefront reductions in a fixed order, and the mixed part keeps per-workgroup
+// partials in LDS, written once per alpha single and summed in ascending order.
 #include "common.h"
 
 #include <algorithm>
@@ -23,102 +25,188 @@ namespace {
 
 constexpr int kB = 256;
 
-__device__ inline double block_reduce(double v, double* sh) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double r = 0.0;
-  if (threadIdx.x == 0)
-    for (int k = 0; k < kB / 64; ++k) r += sh[k];
-  __syncthreads();
-  return r;
+// Sum of v over the 64 lanes of a wavefront in a fixed order (xor butterfly: every lane ends
+// with the same value, the same on every run).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
 }
 
-// one block per row: rowsq[row], t[row * L + q] = sum_col W^2 n_q(col)
-__global__ __launch_bounds__(kB) void k_rowstats(const double* __restrict__ W, int64_t nrow, int64_t ncol,
+// Per-lane sums p and, for the first LQ orbitals, p n_q(col): the bins of a column are its
+// string's occupied orbitals, taken without branches (select) so they stay in registers.
+template <int LQ>
+struct Bins {
+  double b[LQ];
+  __device__ inline void zero() {
+#pragma unroll
+    for (int q = 0; q < LQ; ++q) b[q] = 0.0;
+  }
+  __device__ inline void add(uint64_t str, double p) {
+#pragma unroll
+    for (int q = 0; q < LQ; ++q) b[q] += ((str >> q) & 1ull) ? p : 0.0;
+  }
+};
+
+// One workgroup per row (4 wavefronts over the columns): rowsq[row] = sum_col W^2 and, when t
+// is given, t[row * L + q] = sum_col W^2 n_q(col).  Lane sums -> wavefront sums -> the four
+// wavefronts added in order: no atomics, the same bits on every run.
+template <int LQ>
+__global__ __launch_bounds__(kB) void k_rowstats(const double* __restrict__ W, int64_t ncol,
                                                  const uint64_t* __restrict__ col_str, int L,
                                                  double* __restrict__ rowsq, double* __restrict__ t) {
-  __shared__ double bins[64];
-  __shared__ double sh[kB / 64];
+  __shared__ double part[kB / 64][LQ + 1];
   const int64_t row = blockIdx.x;
-  if (threadIdx.x < 64) bins[threadIdx.x] = 0.0;
-  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double s = 0.0;
+  Bins<LQ> bins;
+  bins.zero();
   for (int64_t col = threadIdx.x; col < ncol; col += kB) {
     const double w = W[row * ncol + col];
     const double w2 = w * w;
     s += w2;
-    if (t) {
-      uint64_t o = col_str[col];
-      while (o) {
-        atomicAdd(&bins[ctz64(o)], w2);
-        o &= o - 1;
+    if (t) bins.add(col_str[col], w2);
+  }
+  s = wave_sum(s);
+  if (lane == 0) part[wv][0] = s;
+  if (t) {
+#pragma unroll
+    for (int q = 0; q < LQ; ++q) {
+      const double v = wave_sum(bins.b[q]);
+      if (lane == 0) part[wv][1 + q] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x <= LQ) {
+    double r = 0.0;
+    for (int k = 0; k < kB / 64; ++k) r += part[k][threadIdx.x];
+    if (threadIdx.x == 0) rowsq[row] = r;
+    else if (t && threadIdx.x - 1 < L) t[row * L + threadIdx.x - 1] = r;
+  }
+}
+
+// One workgroup per row, one wavefront per listed ket (the row's kets dealt round-robin over
+// the four wavefronts): rho_k = sum_col W[row, col] W[ket_k, col] and, with kBins,
+// rho_kq = the same weighted by n_q(col) for q < L.  out[k * (1 + L)] = rho_k,
+// out[k * (1 + L) + 1 + q] = rho_kq (stride 1 without bins).  No barriers, no atomics.
+template <int LQ, bool kBins>
+__global__ __launch_bounds__(kB) void k_pairdots(const double* __restrict__ W, int64_t ncol,
+                                                 const int64_t* __restrict__ off,
+                                                 const uint32_t* __restrict__ ket,
+                                                 const uint64_t* __restrict__ col_str, int L,
+                                                 double* __restrict__ out) {
+  const int64_t row = blockIdx.x;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int stride = kBins ? 1 + L : 1;
+  const double* __restrict__ wr = W + row * ncol;
+  for (int64_t k = off[row] + wv; k < off[row + 1]; k += kB / 64) {
+    const double* __restrict__ wk = W + static_cast<int64_t>(ket[k]) * ncol;
+    double s = 0.0;
+    Bins<kBins ? LQ : 1> bins;
+    bins.zero();
+    for (int64_t col = lane; col < ncol; col += 64) {
+      const double p = wr[col] * wk[col];
+      s += p;
+      if (kBins) bins.add(col_str[col], p);
+    }
+    s = wave_sum(s);
+    if (lane == 0) out[k * stride] = s;
+    if (kBins) {
+#pragma unroll
+      for (int q = 0; q < (kBins ? LQ : 1); ++q) {
+        const double v = wave_sum(bins.b[q]);
+        if (lane == 0 && q < L) out[k * stride + 1 + q] = v;
       }
     }
   }
-  s = block_reduce(s, sh);
-  if (threadIdx.x == 0) rowsq[row] = s;
-  if (t && threadIdx.x < L) t[row * L + threadIdx.x] = bins[threadIdx.x];
 }
 
-// one block per row: for every listed ket k of the row, rho (and rho_q if nq > 0)
-__global__ __launch_bounds__(kB) void k_pairdots(const double* __restrict__ W, int64_t nrow, int64_t ncol,
-                                                 const int64_t* __restrict__ off,
-                                                 const uint32_t* __restrict__ ket,
-                                                 const uint64_t* __restrict__ col_str, int L, int nq,
-                                                 double* __restrict__ out) {
-  __shared__ double bins[64];
-  __shared__ double sh[kB / 64];
-  const int64_t row = blockIdx.x;
-  const int stride = 1 + nq;
-  for (int64_t k = off[row]; k < off[row + 1]; ++k) {
-    if (nq && threadIdx.x < 64) bins[threadIdx.x] = 0.0;
+// Alpha single x beta single part of the 2-RDM:
+//   Z[k][pb] = sum over beta singles (ib -> jb, pair pb, sign s_b) of s_b W[ia, ib] W[ja, jb]
+// for every alpha single k = (ia <- ja) of a chunk of alpha rows, and then
+//   G[pa][pb] = sum over alpha singles k with pair pa of s_a Z[k][pb]   (k_rdm_fold).
+// Beta singles are cut by pair range (kPbr pairs each) and, inside a range, sorted by jb and
+// cut into tiles of kThreads x kEpt entries, so the gathers of W[ja, jb] along a tile touch few
+// lines.  One workgroup = (alpha row ia, pair range r): its tile entries and their bra values
+// W[ia, ib] sit in registers; the row's alpha singles are taken kmax at a time, each with an
+// LDS bin row of the range's pairs (LDS atomic adds: per-workgroup partials; the bins of one
+// alpha single are written once to Z, with no global atomics, and G is summed from Z in
+// ascending k).
+constexpr int kRdmThreads = 512, kRdmEpt = 8;
+constexpr int kRdmTile = kRdmThreads * kRdmEpt;
+constexpr int kRdmBinBytes = 64 * 1024;
+
+__global__ __launch_bounds__(kRdmThreads) void k_rdm_mixed(
+    const double* __restrict__ W, int64_t nb, int64_t row0, int64_t nrow, int L2, int pbr, int kmax,
+    const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket, const int64_t* __restrict__ r_tile,
+    const int64_t* __restrict__ t_off, const uint32_t* __restrict__ e_jb, const uint32_t* __restrict__ e_ib,
+    const uint32_t* __restrict__ e_meta, int64_t k_base, double* __restrict__ Z) {
+  extern __shared__ __align__(16) double bins[];  // [kmax][pbr]
+  const int nrange = static_cast<int>(gridDim.x / nrow);
+  const int64_t ia = row0 + blockIdx.x / nrange;
+  const int r = static_cast<int>(blockIdx.x % nrange);
+  const int tid = threadIdx.x;
+  const int64_t k0 = a_off[ia], k1 = a_off[ia + 1];
+  const int pb0 = r * pbr;
+  const int npb = L2 - pb0 < pbr ? L2 - pb0 : pbr;
+  const double* __restrict__ wbra = W + ia * nb;
+  for (int64_t kc = k0; kc < k1; kc += kmax) {
+    const int nk = static_cast<int>(k1 - kc < kmax ? k1 - kc : kmax);
+    for (int q = tid; q < nk * pbr; q += kRdmThreads) bins[q] = 0.0;
     __syncthreads();
-    const double* wk = W + static_cast<int64_t>(ket[k]) * ncol;
-    const double* wr = W + row * ncol;
-    double s = 0.0;
-    for (int64_t col = threadIdx.x; col < ncol; col += kB) {
-      const double p = wr[col] * wk[col];
-      s += p;
-      if (nq) {
-        uint64_t o = col_str[col];
-        while (o) {
-          atomicAdd(&bins[ctz64(o)], p);
-          o &= o - 1;
+    for (int64_t t = r_tile[r]; t < r_tile[r + 1]; ++t) {
+      const int64_t e0 = t_off[t], e1 = t_off[t + 1];
+      uint32_t jb[kRdmEpt], meta[kRdmEpt];
+      double wb[kRdmEpt];
+#pragma unroll
+      for (int j = 0; j < kRdmEpt; ++j) {
+        const int64_t e = e0 + tid + static_cast<int64_t>(j) * kRdmThreads;
+        const bool ok = e < e1;
+        jb[j] = ok ? e_jb[e] : 0u;
+        meta[j] = ok ? e_meta[e] : 0u;
+        const double w = ok ? wbra[e_ib[e]] : 0.0;
+        wb[j] = meta[j] >> 31 ? -w : w;
+      }
+      for (int q = 0; q < nk; ++q) {
+        const double* __restrict__ xr = W + static_cast<int64_t>(a_ket[kc + q]) * nb;
+        double xv[kRdmEpt];
+#pragma unroll
+        for (int j = 0; j < kRdmEpt; ++j) xv[j] = xr[jb[j]];
+        double* __restrict__ row = bins + q * pbr;
+#pragma unroll
+        for (int j = 0; j < kRdmEpt; ++j) {
+          const double p = wb[j] * xv[j];
+          if (p != 0.0) atomicAdd(row + (meta[j] & 0xffffu), p);
         }
       }
     }
-    s = block_reduce(s, sh);
-    if (threadIdx.x == 0) out[k * stride] = s;
-    if (nq && threadIdx.x < L) out[k * stride + 1 + threadIdx.x] = bins[threadIdx.x];
+    __syncthreads();
+    for (int q = tid; q < nk * pbr; q += kRdmThreads) {
+      const int k = q / pbr, pb = q - k * pbr;
+      if (pb < npb) Z[(kc + k - k_base) * L2 + pb0 + pb] = bins[q];
+    }
     __syncthreads();
   }
 }
 
-// alpha single x beta single products binned by (pa, pb); one thread per determinant
-__global__ __launch_bounds__(kB) void k_mixed(const double* __restrict__ W, int64_t na, int64_t nb, int L,
-                                              const int64_t* __restrict__ aoff, const uint32_t* __restrict__ aket,
-                                              const uint32_t* __restrict__ aexc,
-                                              const int64_t* __restrict__ boff, const uint32_t* __restrict__ bket,
-                                              const uint32_t* __restrict__ bexc, double* __restrict__ G) {
-  const int64_t nt = cdiv(nb, kB);
-  const int64_t ia = blockIdx.x / nt;
-  const int64_t ib = (blockIdx.x % nt) * kB + threadIdx.x;
-  if (ib >= nb) return;
-  const double wb = W[ia * nb + ib];
-  if (wb == 0.0) return;
-  const int64_t L2 = static_cast<int64_t>(L) * L;
-  for (int64_t k = aoff[ia]; k < aoff[ia + 1]; ++k) {
-    const uint32_t e = aexc[k];
-    const int64_t pa = ((e >> 6) & 63) * L + (e & 63);
-    const double sa = (e >> 12) & 1u ? -wb : wb;
-    const double* wr = W + static_cast<int64_t>(aket[k]) * nb;
-    for (int64_t m = boff[ib]; m < boff[ib + 1]; ++m) {
-      const uint32_t f = bexc[m];
-      const int64_t pb = ((f >> 6) & 63) * L + (f & 63);
-      const double v = sa * wr[bket[m]];
-      atomicAdd(&G[pa * L2 + pb], (f >> 12) & 1u ? -v : v);
+// G[pa][pb] += sum over the alpha singles of the chunk with pair pa (ascending k) of
+// s_a Z[k][pb]; one workgroup per pa.
+__global__ __launch_bounds__(kB) void k_rdm_fold(const double* __restrict__ Z, int L2,
+                                                 const int64_t* __restrict__ pa_off,
+                                                 const int64_t* __restrict__ pa_k, int64_t k_base,
+                                                 double* __restrict__ G) {
+  const int64_t pa = blockIdx.x;
+  const int64_t q0 = pa_off[pa], q1 = pa_off[pa + 1];
+  if (q0 == q1) return;
+  for (int pb = threadIdx.x; pb < L2; pb += kB) {
+    double s = 0.0;
+    for (int64_t q = q0; q < q1; ++q) {
+      const int64_t kk = pa_k[q];
+      const int64_t k = kk < 0 ? ~kk : kk;  // negative: s_a = -1
+      const double z = Z[(k - k_base) * L2 + pb];
+      s += kk < 0 ? -z : z;
     }
+    G[pa * L2 + pb] += s;
   }
 }
 
@@ -217,6 +305,122 @@ void two_diff(Rdm& r, int c0, int c1, int d0, int d1, double w) {
   }
 }
 
+// Pair-range tiles of the beta singles for the mixed part (k_rdm_mixed), built on the host
+// from the beta lists: entries cut by pair range, sorted by (jb, ib) inside a range, and cut
+// into tiles of kRdmTile entries.
+struct RdmTiles {
+  int pbr = 0, nrange = 0, kmax = 0;
+  std::vector<int64_t> r_tile, t_off;
+  std::vector<uint32_t> jb, ib, meta;
+};
+
+RdmTiles rdm_tiles(const HostSpin& B, int L, int max_alpha_singles) {
+  RdmTiles T;
+  const int L2 = L * L;
+  T.kmax = std::max(1, std::min(max_alpha_singles, 32));
+  int pbr = std::min<int>(L2, kRdmBinBytes / (8 * T.kmax));
+  T.nrange = static_cast<int>(cdiv(L2, pbr));
+  T.pbr = static_cast<int>(cdiv(L2, T.nrange));
+  struct Ent {
+    uint32_t r, jb, ib, meta;
+  };
+  std::vector<Ent> ent;
+  const int64_t nb = static_cast<int64_t>(B.str.size());
+  ent.reserve(B.ket_s.size());
+  for (int64_t ib = 0; ib < nb; ++ib)
+    for (int64_t k = B.off_s[ib]; k < B.off_s[ib + 1]; ++k) {
+      const uint32_t e = B.exc_s[k];
+      const uint32_t pb = ((e >> 6) & 63) * static_cast<uint32_t>(L) + (e & 63);
+      const uint32_t r = pb / static_cast<uint32_t>(T.pbr);
+      ent.push_back(Ent{r, B.ket_s[k], static_cast<uint32_t>(ib),
+                        (pb - r * static_cast<uint32_t>(T.pbr)) | (((e >> 12) & 1u) << 31)});
+    }
+  std::sort(ent.begin(), ent.end(), [](const Ent& u, const Ent& v) {
+    return u.r != v.r ? u.r < v.r : (u.jb != v.jb ? u.jb < v.jb : u.ib < v.ib);
+  });
+  T.jb.resize(ent.size());
+  T.ib.resize(ent.size());
+  T.meta.resize(ent.size());
+  for (size_t q = 0; q < ent.size(); ++q) {
+    T.jb[q] = ent[q].jb;
+    T.ib[q] = ent[q].ib;
+    T.meta[q] = ent[q].meta;
+  }
+  T.t_off.push_back(0);
+  size_t q = 0;
+  for (int r = 0; r < T.nrange; ++r) {
+    T.r_tile.push_back(static_cast<int64_t>(T.t_off.size()) - 1);
+    size_t end = q;
+    while (end < ent.size() && static_cast<int>(ent[end].r) == r) ++end;
+    while (q < end) {
+      q = std::min(end, q + static_cast<size_t>(kRdmTile));
+      T.t_off.push_back(static_cast<int64_t>(q));
+    }
+  }
+  T.r_tile.push_back(static_cast<int64_t>(T.t_off.size()) - 1);
+  return T;
+}
+
+template <class T>
+DevBuf<T> to_dev(const std::vector<T>& h, hipStream_t st) {
+  DevBuf<T> d(std::max<size_t>(h.size(), 1));
+  if (!h.empty()) SBDX_HIP(hipMemcpy(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+// G[pa][pb] (pa = a L + i, pb = b L + j) of the alpha single x beta single part, on the device.
+void rdm_mixed(sbdx_ctx& c, const double* dW, const HostSpin& A, const HostSpin& B, double* G) {
+  const int L = c.L, L2 = L * L;
+  const int64_t na = c.na, nb = c.nb;
+  hipStream_t st = c.stream;
+  if (c.A.nnz_s == 0 || c.B.nnz_s == 0) return;
+  const RdmTiles T = rdm_tiles(B, L, c.A.max_s);
+  auto d_rt = to_dev(T.r_tile, st);
+  auto d_to = to_dev(T.t_off, st);
+  auto d_jb = to_dev(T.jb, st);
+  auto d_ib = to_dev(T.ib, st);
+  auto d_meta = to_dev(T.meta, st);
+  // alpha rows in chunks whose Z (alpha singles x L^2 FP64) stays under zcap bytes
+  const int64_t zcap = int64_t(6) << 30;
+  const int64_t per_single = static_cast<int64_t>(L2) * 8;
+  int64_t max_rows_singles = 0;
+  for (int64_t ia = 0; ia < na; ++ia) max_rows_singles = std::max(max_rows_singles, A.off_s[ia + 1] - A.off_s[ia]);
+  const int64_t zk = std::max<int64_t>(max_rows_singles, std::min<int64_t>(c.A.nnz_s, zcap / per_single));
+  DevBuf<double> Z(zk * L2);
+  const size_t lds = static_cast<size_t>(T.kmax) * T.pbr * 8;
+  int64_t row0 = 0;
+  while (row0 < na) {
+    int64_t row1 = row0;
+    while (row1 < na && A.off_s[row1 + 1] - A.off_s[row0] <= zk) ++row1;
+    const int64_t k_base = A.off_s[row0], nk = A.off_s[row1] - k_base;
+    if (nk > 0) {
+      // the chunk's alpha singles grouped by pair pa, ascending k; ~k marks s_a = -1
+      std::vector<int64_t> pa_off(L2 + 1, 0), pa_k(nk);
+      for (int64_t k = k_base; k < k_base + nk; ++k) {
+        const uint32_t e = A.exc_s[k];
+        ++pa_off[((e >> 6) & 63) * L + (e & 63) + 1];
+      }
+      for (int p = 0; p < L2; ++p) pa_off[p + 1] += pa_off[p];
+      std::vector<int64_t> fill(pa_off.begin(), pa_off.end() - 1);
+      for (int64_t k = k_base; k < k_base + nk; ++k) {
+        const uint32_t e = A.exc_s[k];
+        pa_k[fill[((e >> 6) & 63) * L + (e & 63)]++] = (e >> 12) & 1u ? ~k : k;
+      }
+      auto d_pa_off = to_dev(pa_off, st);
+      auto d_pa_k = to_dev(pa_k, st);
+      const int64_t nrow = row1 - row0;
+      k_rdm_mixed<<<static_cast<unsigned>(nrow * T.nrange), kRdmThreads, lds, st>>>(
+          dW, nb, row0, nrow, L2, T.pbr, T.kmax, c.A.off_s.get(), c.A.ket_s.get(), d_rt.get(), d_to.get(),
+          d_jb.get(), d_ib.get(), d_meta.get(), k_base, Z.get());
+      SBDX_CHECK_LAUNCH();
+      k_rdm_fold<<<static_cast<unsigned>(L2), kB, 0, st>>>(Z.get(), L2, d_pa_off.get(), d_pa_k.get(), k_base, G);
+      SBDX_CHECK_LAUNCH();
+      SBDX_HIP(hipStreamSynchronize(st));  // host vectors of this chunk go out of scope
+    }
+    row0 = row1;
+  }
+}
+
 }  // namespace
 
 void occupation_blocked(sbdx_ctx& c, const double* dW, double* density) {
@@ -224,11 +428,11 @@ void occupation_blocked(sbdx_ctx& c, const double* dW, double* density) {
   const int64_t na = c.na, nb = c.nb, n = c.ndet();
   hipStream_t st = c.stream;
   DevBuf<double> rowsq(na), colsq(nb), wT(n);
-  k_rowstats<<<static_cast<unsigned>(na), kB, 0, st>>>(dW, na, nb, c.B.str.get(), L, rowsq.get(), nullptr);
+  k_rowstats<1><<<static_cast<unsigned>(na), kB, 0, st>>>(dW, nb, c.B.str.get(), L, rowsq.get(), nullptr);
   SBDX_CHECK_LAUNCH();
   k_transpose_d<double><<<static_cast<unsigned>(cdiv(na, 64) * cdiv(nb, 64)), 256, 0, st>>>(dW, wT.get(), na, nb);
   SBDX_CHECK_LAUNCH();
-  k_rowstats<<<static_cast<unsigned>(nb), kB, 0, st>>>(wT.get(), nb, na, c.A.str.get(), L, colsq.get(), nullptr);
+  k_rowstats<1><<<static_cast<unsigned>(nb), kB, 0, st>>>(wT.get(), na, c.A.str.get(), L, colsq.get(), nullptr);
   SBDX_CHECK_LAUNCH();
   SBDX_HIP(hipStreamSynchronize(st));
   auto ra = to_host(rowsq, na), cb = to_host(colsq, nb);
@@ -254,21 +458,24 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
   SBDX_HIP(hipMemsetAsync(G.get(), 0, L4 * 8, st));
   k_transpose_d<double><<<static_cast<unsigned>(cdiv(na, 64) * cdiv(nb, 64)), 256, 0, st>>>(dW, wT.get(), na, nb);
   SBDX_CHECK_LAUNCH();
-  k_rowstats<<<static_cast<unsigned>(na), kB, 0, st>>>(dW, na, nb, c.B.str.get(), L, rowsq.get(), ta.get());
-  k_rowstats<<<static_cast<unsigned>(nb), kB, 0, st>>>(wT.get(), nb, na, c.A.str.get(), L, colsq.get(), nullptr);
-  SBDX_CHECK_LAUNCH();
-  k_pairdots<<<static_cast<unsigned>(na), kB, 0, st>>>(dW, na, nb, c.A.off_s.get(), c.A.ket_s.get(), c.B.str.get(), L, L, rs_a.get());
-  k_pairdots<<<static_cast<unsigned>(na), kB, 0, st>>>(dW, na, nb, c.A.off_d.get(), c.A.ket_d.get(), c.B.str.get(), L, 0, rd_a.get());
-  k_pairdots<<<static_cast<unsigned>(nb), kB, 0, st>>>(wT.get(), nb, na, c.B.off_s.get(), c.B.ket_s.get(), c.A.str.get(), L, L, rs_b.get());
-  k_pairdots<<<static_cast<unsigned>(nb), kB, 0, st>>>(wT.get(), nb, na, c.B.off_d.get(), c.B.ket_d.get(), c.A.str.get(), L, 0, rd_b.get());
-  SBDX_CHECK_LAUNCH();
-  k_mixed<<<static_cast<unsigned>(na * cdiv(nb, kB)), kB, 0, st>>>(
-      dW, na, nb, L, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), c.B.off_s.get(), c.B.ket_s.get(),
-      c.B.exc_s.get(), G.get());
-  SBDX_CHECK_LAUNCH();
-  SBDX_HIP(hipStreamSynchronize(st));
-
+  auto stats = [&](auto lq) {
+    constexpr int LQ = decltype(lq)::value;
+    const unsigned ga = static_cast<unsigned>(na), gb = static_cast<unsigned>(nb);
+    k_rowstats<LQ><<<ga, kB, 0, st>>>(dW, nb, c.B.str.get(), L, rowsq.get(), ta.get());
+    k_rowstats<1><<<gb, kB, 0, st>>>(wT.get(), na, c.A.str.get(), L, colsq.get(), nullptr);
+    SBDX_CHECK_LAUNCH();
+    k_pairdots<LQ, true><<<ga, kB, 0, st>>>(dW, nb, c.A.off_s.get(), c.A.ket_s.get(), c.B.str.get(), L, rs_a.get());
+    k_pairdots<1, false><<<ga, kB, 0, st>>>(dW, nb, c.A.off_d.get(), c.A.ket_d.get(), c.B.str.get(), L, rd_a.get());
+    k_pairdots<LQ, true><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_s.get(), c.B.ket_s.get(), c.A.str.get(), L, rs_b.get());
+    k_pairdots<1, false><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_d.get(), c.B.ket_d.get(), c.A.str.get(), L, rd_b.get());
+    SBDX_CHECK_LAUNCH();
+  };
+  if (L <= 32) stats(std::integral_constant<int, 32>{});
+  else if (L <= 48) stats(std::integral_constant<int, 48>{});
+  else stats(std::integral_constant<int, 64>{});
   const HostSpin A = host_spin(c.A), B = host_spin(c.B);
+  rdm_mixed(c, dW, A, B, G.get());
+  SBDX_HIP(hipStreamSynchronize(st));
   auto hr = to_host(rowsq, na), hc = to_host(colsq, nb), ht = to_host(ta, na * L);
   auto hsa = to_host(rs_a, c.A.nnz_s * (L + 1)), hda = to_host(rd_a, c.A.nnz_d);
   auto hsb = to_host(rs_b, c.B.nnz_s * (L + 1)), hdb = to_host(rd_b, c.B.nnz_d);

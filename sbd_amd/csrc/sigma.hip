@@ -114,6 +114,35 @@ __device__ inline void stk(T* p, const VecK<T, K>& a) {
   *reinterpret_cast<VecK<T, K>*>(p) = a;
 }
 
+// Raw buffer resource over `bytes` bytes at p (stride 0; dword 3 = 0x00020000, the GFX9-family
+// raw-buffer format): a load addresses it with a 32-bit per-lane byte offset and no 64-bit
+// address arithmetic, and a load past `bytes` returns zeros.
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), static_cast<short>(0), static_cast<int>(bytes),
+                                           0x00020000);
+}
+
+// K contiguous values at byte offset off of a buffer resource
+template <typename T, int K>
+__device__ inline VecK<T, K> ldk_buf(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+  constexpr int B = K * static_cast<int>(sizeof(T));
+  VecK<T, K> a;
+  if constexpr (B == 4) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    __builtin_memcpy(&a, &v, 4);
+  } else if constexpr (B == 8) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+    __builtin_memcpy(&a, &v, 8);
+  } else {
+#pragma unroll
+    for (int c = 0; c < B / 16; ++c) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * c, 0, 0);
+      __builtin_memcpy(reinterpret_cast<char*>(&a) + 16 * c, &v, 16);
+    }
+  }
+  return a;
+}
+
 struct RowTabs {
   const int64_t* off_s;
   const uint32_t* ket_s;
@@ -125,20 +154,21 @@ struct RowTabs {
 // Same-spin part (and, for the alpha rows, the diagonal): y[row, col] = sum over the row's
 // singles and doubles of value * x[ket, col], for K interleaved vectors.  One workgroup = one
 // row x kRowTile columns: the row's list entries are the same for every lane (scalar loads),
-// the gathers of one entry are contiguous across the lanes.
+// the gathers of one entry are contiguous across the lanes.  A single's value is its stored
+// same-spin part plus the spectator string's Coulomb term, read from the Coulomb table row of
+// its orbital pair (coalesced across the lanes, in flight with the gathers).
 template <typename T, bool kDiag, int K, int kAcc>
 __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
     int64_t nrow, int64_t ncol, int L, RowTabs rt, const T* __restrict__ val_s,
-    const T* __restrict__ val_d, const uint64_t* __restrict__ col_str, const T* __restrict__ V,
-    const T* __restrict__ x, T* __restrict__ y, const T* __restrict__ hii) {
-  // gathers issued back to back per wavefront before their FMAs
-  constexpr int U = K == 1 ? 8 : (K == 2 ? 4 : 2);
+    const T* __restrict__ val_d, const T* __restrict__ coul, const T* __restrict__ x, T* __restrict__ y,
+    const T* __restrict__ hii) {
+  // gathers issued back to back per wavefront before their FMAs: 16 of up to 8 bytes, 4 of 16
+  constexpr int U = K * sizeof(T) <= 8 ? 16 : (K * sizeof(T) == 16 ? 4 : 2);
   const int64_t g = blockIdx.x;
   const int64_t tile = g / nrow;
   const int64_t row = g - tile * nrow;
   const int64_t col = tile * kRowTile + threadIdx.x;
   if (col >= ncol) return;
-  const int64_t L2 = static_cast<int64_t>(L) * L;
   typename AccOf<T, kAcc>::type acc[K];
   const int64_t me = row * ncol + col;
   if (kDiag) {
@@ -171,36 +201,33 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
     for (int v = 0; v < K; ++v) acc[v].fma(w, xv.v[v]);
   }
 
-  // same-spin singles, with the spectator string's Coulomb term sum_{j in col string} (ai|jj)
-  const uint64_t ycol = col_str[col];
+  // same-spin singles: value = stored part + sign * Coulomb table [pair, col]
   const int64_t s0 = rt.off_s[row], s1 = rt.off_s[row + 1];
-  auto single_value = [&](int64_t q) {
+  const T* __restrict__ cc = coul + col;
+  auto pair_of = [&](int64_t q) {
     const uint32_t e = rt.exc_s[q];
-    const T* vp = V + (static_cast<int64_t>((e >> 6) & 63) * L + (e & 63)) * L2;
-    T coul = T(0);
-    uint64_t o = ycol;
-    while (o) {
-      const int j = ctz64(o);
-      o &= o - 1;
-      coul += vp[j * L + j];
-    }
-    return val_s[q] + ((e >> 12) & 1u ? -coul : coul);
+    return static_cast<int64_t>((e >> 6) & 63) * L + (e & 63);
   };
+  auto single_value = [&](int64_t q, T d) { return val_s[q] + ((rt.exc_s[q] >> 12) & 1u ? -d : d); };
   k = s0;
   for (; k + U <= s1; k += U) {
     VecK<T, K> xv[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_s[k + u]) * stride);
+    T dv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const T w = single_value(k + u);
+      xv[u] = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_s[k + u]) * stride);
+      dv[u] = cc[pair_of(k + u) * ncol];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const T w = single_value(k + u, dv[u]);
 #pragma unroll
       for (int v = 0; v < K; ++v) acc[v].fma(w, xv[u].v[v]);
     }
   }
   for (; k < s1; ++k) {
     const VecK<T, K> xv = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_s[k]) * stride);
-    const T w = single_value(k);
+    const T w = single_value(k, cc[pair_of(k) * ncol]);
 #pragma unroll
     for (int v = 0; v < K; ++v) acc[v].fma(w, xv.v[v]);
   }
@@ -217,19 +244,23 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
 // loaded once per workgroup; for every alpha single the integral row V[(a_k i_k), :] (sign
 // folded in) is staged in double-buffered LDS, and the ket row x[ja_k, :] is gathered along
 // sorted jb.  Workgroups are numbered so that all tiles of one row run on the same XCD and
-// share that row's ket rows in L2.  At the end every column sums its entries' partial sums in
-// ascending entry position (a fixed order: sigma is bitwise reproducible), and sigma is
-// updated by its owner, with no atomics.
+// share that row's ket rows in L2.  At the end every entry writes its sum to LDS at its slot in
+// column order (the beta list's own order: column by column, kets ascending), and every column
+// sums its contiguous run of slots in that order (a fixed order: sigma is bitwise
+// reproducible); sigma is updated by its owner, with no atomics.
 constexpr int kMixMaxCols = 1024;
-// Bytes of LDS the end-of-kernel fold stages per group of entries (the integral-row buffers
-// are reused for it); a group is kFoldP entries per thread.
-constexpr int kFoldBytes = 16384;
-__host__ __device__ constexpr int fold_group(int ept, int threads, int k, int wide) {
-  int p = ept;
-  while (p > 1 && p * threads * k * wide > kFoldBytes) p /= 2;
-  return p;
-}
 constexpr size_t kMixMaxLds = 160 * 1024;  // LDS of one CU
+// LDS bytes of the row's alpha-single list (ket row and packed word per single), 16-aligned
+__host__ __device__ constexpr size_t mix_list_bytes(int max_s) {
+  return (static_cast<size_t>(max_s) * 8 + 15) / 16 * 16;
+}
+// LDS bytes of a variant: the row's alpha-single list and the double-buffered integral rows of
+// its stages, reused at the end for one vector's entry sums (tile entries x accumulator width).
+__host__ __device__ constexpr size_t mix_lds_bytes(int L, int max_s, int stages, int elem, int tile, int wide) {
+  const size_t main = mix_list_bytes(max_s) + 2 * static_cast<size_t>(stages) * L * L * elem;
+  const size_t fold = static_cast<size_t>(tile) * wide;
+  return main > fold ? main : fold;
+}
 // Variants of the kernel: entries per thread (kEPT), threads per workgroup (kMixThreads) and
 // alpha singles staged per barrier (kStage), coded as 1000000 (kStage - 1) + 100 * threads +
 // entries per thread.  Every workgroup of a row gathers from all of
@@ -243,15 +274,19 @@ constexpr size_t kMixMaxLds = 160 * 1024;  // LDS of one CU
 
 template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage, int kAcc>
 __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
-    int64_t na, int64_t nb, int L, const int64_t* __restrict__ a_off, const uint32_t* __restrict__ a_ket,
-    const uint32_t* __restrict__ a_exc, int ntile, const int64_t* __restrict__ t_off,
-    const int32_t* __restrict__ t_col0, const int32_t* __restrict__ t_ncol,
+    int64_t na, int64_t nb, int L, int max_s, const int64_t* __restrict__ a_off,
+    const uint32_t* __restrict__ a_ket, const uint32_t* __restrict__ a_exc, int ntile,
+    const int64_t* __restrict__ t_off, const int32_t* __restrict__ t_col0, const int32_t* __restrict__ t_ncol,
     const uint32_t* __restrict__ e_jb, const uint32_t* __restrict__ e_meta,
-    const int64_t* __restrict__ b_off, const uint16_t* __restrict__ e_pos, const T* __restrict__ V,
-    const T* __restrict__ x, T* __restrict__ y) {
+    const int64_t* __restrict__ b_off, const T* __restrict__ V, const T* __restrict__ x,
+    T* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L2 = L * L;
-  T* vs = reinterpret_cast<T*>(smem_raw);  // [2][kStage][L2]: double-buffered V rows
+  // the row's alpha singles (ket row, packed excitation word), then the double-buffered V rows
+  // [2][kStage][L2]; the singles are read from LDS, so their loads never wait on the gathers
+  uint32_t* s_ket = reinterpret_cast<uint32_t*>(smem_raw);
+  uint32_t* s_exc = s_ket + max_s;
+  T* vs = reinterpret_cast<T*>(smem_raw + mix_list_bytes(max_s));
 
   // XCD-aware numbering: workgroup g runs on XCD g % 8; XCD x takes rows ia = x (mod 8), all
   // tiles of a row back to back
@@ -261,33 +296,40 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
   const int64_t ia = xcd + 8 * (slot / ntile);
   if (ia >= na) return;
   const int tid = threadIdx.x;
-  const int64_t k0 = a_off[ia], k1 = a_off[ia + 1];
-  if (k0 == k1) return;
+  const int64_t k0 = a_off[ia];
+  const int nk = static_cast<int>(a_off[ia + 1] - k0);
+  if (nk == 0) return;
+  for (int t = tid; t < nk; t += kMixThreads) {
+    s_ket[t] = a_ket[k0 + t];
+    s_exc[t] = a_exc[k0 + t];
+  }
 
   const int64_t e0 = t_off[tile], e1 = t_off[tile + 1];
   const int col0 = t_col0[tile], ncol = t_ncol[tile];
-  uint32_t jb[kEPT], meta[kEPT];
+  uint32_t joff[kEPT], meta[kEPT];
   typename AccOf<T, kAcc>::type acc[kEPT][K];
   // padding entries (past the tile's end) read column 0 and pair 0, which are always valid, so
   // the entry loop below has no branches and all its gathers can be in flight at once; their
-  // sums are dropped at the end (ib_local 0xffff)
+  // sums are dropped at the end (slot 0xffff).  joff: byte offset of the entry's ket column in
+  // a ket row
 #pragma unroll
   for (int j = 0; j < kEPT; ++j) {
     const int64_t e = e0 + tid + static_cast<int64_t>(j) * kMixThreads;
     const bool ok = e < e1;
-    jb[j] = ok ? e_jb[e] : 0u;
+    joff[j] = (ok ? e_jb[e] : 0u) * static_cast<uint32_t>(K * sizeof(T));
     meta[j] = ok ? e_meta[e] : 0x0000ffffu;
   }
+  __syncthreads();  // the row's singles are staged
 
   // V rows of the next stage's alpha singles: fetched into registers before the current
   // stage's gathers (their latency hides under them), written to LDS after them
   T pre[kStage][kVPerThread];
   bool pre_neg[kStage];
-  auto fetch = [&](int64_t kk) {
+  auto fetch = [&](int kk) {
 #pragma unroll
     for (int q = 0; q < kStage; ++q) {
-      if (kk + q >= k1) break;
-      const uint32_t e = a_exc[kk + q];
+      if (kk + q >= nk) break;
+      const uint32_t e = __builtin_amdgcn_readfirstlane(s_exc[kk + q]);
       const T* src = V + static_cast<int64_t>(((e >> 6) & 63) * L + (e & 63)) * L2;
       pre_neg[q] = (e >> 12) & 1u;
 #pragma unroll
@@ -297,10 +339,10 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
       }
     }
   };
-  auto commit = [&](T* dst, int64_t kk) {
+  auto commit = [&](T* dst, int kk) {
 #pragma unroll
     for (int q = 0; q < kStage; ++q) {
-      if (kk + q >= k1) break;
+      if (kk + q >= nk) break;
 #pragma unroll
       for (int u = 0; u < kVPerThread; ++u) {
         const int p = tid + u * kMixThreads;
@@ -308,26 +350,29 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
       }
     }
   };
-  fetch(k0);
-  commit(vs, k0);
+  fetch(0);
+  commit(vs, 0);
+  const int64_t row_bytes = nb * K * static_cast<int64_t>(sizeof(T));
   int buf = 0;
-  for (int64_t kk = k0; kk < k1; kk += kStage, buf ^= 1) {
+  for (int kk = 0; kk < nk; kk += kStage, buf ^= 1) {
     T* cur = vs + buf * kStage * L2;
     T* nxt = vs + (buf ^ 1) * kStage * L2;
     __syncthreads();  // cur is staged; nobody reads nxt any more
-    const bool more = kk + kStage < k1;
+    const bool more = kk + kStage < nk;
     if (more) fetch(kk + kStage);
     VecK<T, K> xv[kStage][kEPT];
 #pragma unroll
     for (int q = 0; q < kStage; ++q) {
-      if (kk + q >= k1) break;
-      const T* __restrict__ xr = x + static_cast<int64_t>(a_ket[kk + q]) * nb * K;
+      if (kk + q >= nk) break;
+      const uint32_t ja = __builtin_amdgcn_readfirstlane(s_ket[kk + q]);
+      const __amdgpu_buffer_rsrc_t xr =
+          buf_rsrc(reinterpret_cast<const unsigned char*>(x) + ja * row_bytes, static_cast<uint32_t>(row_bytes));
 #pragma unroll
-      for (int j = 0; j < kEPT; ++j) xv[q][j] = ldk<T, K>(xr + static_cast<int64_t>(jb[j]) * K);
+      for (int j = 0; j < kEPT; ++j) xv[q][j] = ldk_buf<T, K>(xr, joff[j]);
     }
 #pragma unroll
     for (int q = 0; q < kStage; ++q) {
-      if (kk + q >= k1) break;
+      if (kk + q >= nk) break;
 #pragma unroll
       for (int j = 0; j < kEPT; ++j) {
         const T w = cur[q * L2 + ((meta[j] >> 16) & 0xfff)];
@@ -338,56 +383,41 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     if (more) commit(nxt, kk + kStage);
   }
 
-  // Fold.  Entry sums pass through LDS (over the integral-row buffers) kFoldP entries per
-  // thread at a time; entry position = tid + j * kMixThreads.  Column c of the tile owns the
-  // beta singles b_off[col0 + c] .. b_off[col0 + c + 1] - 1, whose positions e_pos lists in
-  // ascending order, and sums them in that order.
+  // Fold, one vector at a time: every entry writes its sum (sign of the beta single applied)
+  // to LDS at its slot = its index in the beta list minus the tile's first index, so column c
+  // of the tile owns the contiguous slots b_off[col0 + c] - b_off[col0] ..
+  // b_off[col0 + c + 1] - b_off[col0] - 1 and sums them in ascending order.
   using Acc = typename AccOf<T, kAcc>::type;
   using Wide = typename Acc::Wide;
-  constexpr int kFoldP = fold_group(kEPT, kMixThreads, K, static_cast<int>(sizeof(Wide)));
   constexpr int kCols = kMixMaxCols / kMixThreads;
   Wide* stage = reinterpret_cast<Wide*>(smem_raw);
-  int64_t cur[kCols], cend[kCols];
-  Acc col[kCols][K];
+  const int64_t base = b_off[col0];
+  int cs[kCols], ce[kCols];
 #pragma unroll
   for (int u = 0; u < kCols; ++u) {
     const int c = tid + u * kMixThreads;
-    cur[u] = c < ncol ? b_off[col0 + c] : 0;
-    cend[u] = c < ncol ? b_off[col0 + c + 1] : 0;
-  }
-#pragma unroll
-  for (int j0 = 0; j0 < kEPT; j0 += kFoldP) {
-    __syncthreads();  // the LDS is free: integral rows (first group) or the previous group
-#pragma unroll
-    for (int jj = 0; jj < kFoldP; ++jj) {
-      const bool neg = meta[j0 + jj] >> 31;
-#pragma unroll
-      for (int v = 0; v < K; ++v) {
-        const Wide w = acc[j0 + jj][v].wide();
-        stage[(jj * kMixThreads + tid) * K + v] = neg ? -w : w;
-      }
-    }
-    __syncthreads();
-    const uint32_t lo = static_cast<uint32_t>(j0) * kMixThreads;
-    const uint32_t hi = lo + static_cast<uint32_t>(kFoldP) * kMixThreads;
-#pragma unroll
-    for (int u = 0; u < kCols; ++u) {
-      while (cur[u] < cend[u]) {
-        const uint32_t p = e_pos[cur[u]];
-        if (p >= hi) break;
-#pragma unroll
-        for (int v = 0; v < K; ++v) col[u][v].add(stage[(p - lo) * K + v]);
-        ++cur[u];
-      }
-    }
+    cs[u] = c < ncol ? static_cast<int>(b_off[col0 + c] - base) : 0;
+    ce[u] = c < ncol ? static_cast<int>(b_off[col0 + c + 1] - base) : 0;
   }
   T* yr = y + (ia * nb + col0) * K;
 #pragma unroll
-  for (int u = 0; u < kCols; ++u) {
-    const int c = tid + u * kMixThreads;
-    if (c < ncol) {
+  for (int v = 0; v < K; ++v) {
+    __syncthreads();  // the LDS is free: integral rows (first vector) or the previous vector
 #pragma unroll
-      for (int v = 0; v < K; ++v) yr[c * K + v] = col[u][v].apply(yr[c * K + v]);
+    for (int j = 0; j < kEPT; ++j) {
+      const uint32_t slot = meta[j] & 0xffffu;
+      const Wide w = acc[j][v].wide();
+      if (slot != 0xffffu) stage[slot] = meta[j] >> 31 ? -w : w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kCols; ++u) {
+      const int c = tid + u * kMixThreads;
+      if (c < ncol) {
+        Acc col;
+        for (int q = cs[u]; q < ce[u]; ++q) col.add(stage[q]);
+        yr[c * K + v] = col.apply(yr[c * K + v]);
+      }
     }
   }
 }
@@ -467,8 +497,8 @@ int mix_variant(const sbdx_ctx& c, int prec, int K) {
     if (K == 1) return 25616;
     if (K == 2) return 25608;
   }
-  if (K == 1) return prec == 0 ? 1102408 : 25632;
-  if (K == 2) return prec == 0 ? 51216 : 1051208;
+  if (K == 1) return prec == 0 ? 1102408 : 102408;
+  if (K == 2) return prec == 0 ? 102408 : 25616;
   if (K == 4) return prec == 0 ? 1102404 : 25608;
   return 25604;
 }
@@ -504,10 +534,8 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   std::vector<uint32_t> e_jb, e_meta;
   e_jb.reserve(B.nnz_s);
   e_meta.reserve(B.nnz_s);
-  std::vector<uint16_t> pos(B.nnz_s);
   struct Ent {
     uint32_t jb, meta;
-    int64_t k;
   };
   std::vector<Ent> buf;
   int64_t ib = 0;
@@ -519,23 +547,21 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
       ++ib;
     }
     if (ib == start) throw Error(-1, "beta string with more singles than a mixed tile holds");
-    if (cap > 65536) throw Error(-1, "mixed tile larger than its 16-bit entry positions");
+    if (cap > 65535) throw Error(-1, "mixed tile larger than its 16-bit entry slots");
     buf.clear();
     for (int64_t b = start; b < ib; ++b)
       for (int64_t k = off[b]; k < off[b + 1]; ++k) {
         const uint32_t e = exc[k];
         const uint32_t pb = ((e >> 6) & 63) * static_cast<uint32_t>(L) + (e & 63);
-        const uint32_t meta = static_cast<uint32_t>(b - start) | (pb << 16) | (((e >> 12) & 1u) << 31);
-        buf.push_back(Ent{ket[k], meta, k});
+        const uint32_t slot = static_cast<uint32_t>(k - off[start]);
+        const uint32_t meta = slot | (pb << 16) | (((e >> 12) & 1u) << 31);
+        buf.push_back(Ent{ket[k], meta});
       }
     std::stable_sort(buf.begin(), buf.end(), [](const Ent& u, const Ent& v) { return u.jb < v.jb; });
     for (size_t r = 0; r < buf.size(); ++r) {
       e_jb.push_back(buf[r].jb);
       e_meta.push_back(buf[r].meta);
-      pos[buf[r].k] = static_cast<uint16_t>(r);
     }
-    // each column's positions ascending: the order in which the kernel sums them
-    for (int64_t b = start; b < ib; ++b) std::sort(pos.begin() + off[b], pos.begin() + off[b + 1]);
     t_col0.push_back(static_cast<int32_t>(start));
     t_ncol.push_back(static_cast<int32_t>(ib - start));
     t_off.push_back(static_cast<int64_t>(e_jb.size()));
@@ -547,13 +573,11 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   mt.ncol.alloc(t_ncol.size());
   mt.jb.alloc(e_jb.size());
   mt.meta.alloc(e_meta.size());
-  mt.pos.alloc(pos.size());
   SBDX_HIP(hipMemcpy(mt.ent_off.get(), t_off.data(), t_off.size() * 8, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.col0.get(), t_col0.data(), t_col0.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.ncol.get(), t_ncol.data(), t_ncol.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.jb.get(), e_jb.data(), e_jb.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.meta.get(), e_meta.data(), e_meta.size() * 4, hipMemcpyHostToDevice));
-  SBDX_HIP(hipMemcpy(mt.pos.get(), pos.data(), pos.size() * 2, hipMemcpyHostToDevice));
   return mt;
 }
 
@@ -575,18 +599,17 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
   using Wide = typename AccOf<T, kAcc>::type::Wide;
   auto launch = [&](auto kernel, int threads, int ept) {
-    // LDS: the double-buffered integral rows, reused at the end by the fold's entry groups
-    const size_t vrows = 2 * static_cast<size_t>(stages) * c.L * c.L * sizeof(T);
-    const size_t fold = static_cast<size_t>(fold_group(ept, threads, K, static_cast<int>(sizeof(Wide)))) *
-                        threads * K * sizeof(Wide);
-    const size_t lds = std::max(vrows, fold);
+    const size_t lds = mix_lds_bytes(c.L, c.A.max_s, stages, static_cast<int>(sizeof(T)), threads * ept,
+                                     static_cast<int>(sizeof(Wide)));
+    if (c.nb * K * static_cast<int64_t>(sizeof(T)) >= (int64_t(1) << 32))
+      throw Error(SBDX_ERR_ARG, "beta string count too large for 32-bit ket-row offsets of the mixed kernel");
     if (lds > kMixMaxLds) throw Error(SBDX_ERR_ARG, "mixed-kernel variant needs more LDS than a CU has for this problem");
     if (lds > 65536)
       SBDX_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     kernel<<<static_cast<unsigned>(blocks), threads, lds, st>>>(
-        c.na, c.nb, c.L, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
-        mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), c.B.off_s.get(), mt.pos.get(), V, x, y);
+        c.na, c.nb, c.L, c.A.max_s, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
+        mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), c.B.off_s.get(), V, x, y);
     SBDX_CHECK_LAUNCH();
   };
   const bool small_v = static_cast<int>(c.L) * c.L <= 2048;
@@ -622,7 +645,9 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
     else if (var == 51216) pick(I16{}, T512{}, S1{});
     else if (var == 102408) pick(I8{}, T1024{}, S1{});
     else if (var == 1102408) pick(I8{}, T1024{}, S2{});
+    else if (var == 3102408) pick(I8{}, T1024{}, S4{});
     else if (var == 1051216) pick(I16{}, T512{}, S2{});
+    else if (var == 1025616) pick(I16{}, T256{}, S2{});
     else throw Error(SBDX_ERR_ARG, "unknown mixed-kernel variant");
   } else if constexpr (K == 2) {
     if (var == 102404) pick(I4{}, T1024{}, S1{});
@@ -649,7 +674,7 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
 
 template <typename T, int K, int kAcc>
 void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const T* vd_a,
-             const T* vs_b, const T* vd_b, const T* V, const T* hii) {
+             const T* vs_b, const T* vd_b, const T* V, const T* hii, const T* coul_a, const T* coul_b) {
   const int64_t na = c.na, nb = c.nb;
   const int64_t n = na * nb;
   RowTabs ta{c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), c.A.off_d.get(), c.A.ket_d.get()};
@@ -661,7 +686,7 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   mark(0);
   // alpha rows: diagonal + alpha singles and doubles
   k_sigma_rows<T, true, K, kAcc><<<static_cast<unsigned>(na * cdiv(nb, kRowTile)), kRowTile, 0, st>>>(
-      na, nb, c.L, ta, vs_a, vd_a, c.B.str.get(), V, x, y, hii);
+      na, nb, c.L, ta, vs_a, vd_a, coul_a, x, y, hii);
   SBDX_CHECK_LAUNCH();
   mark(1);
   // alpha single x beta single
@@ -678,7 +703,7 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   SBDX_CHECK_LAUNCH();
   mark(3);
   k_sigma_rows<T, false, K, kAcc><<<static_cast<unsigned>(nb * cdiv(na, kRowTile)), kRowTile, 0, st>>>(
-      nb, na, c.L, tb, vs_b, vd_b, c.A.str.get(), V, xT, yT, nullptr);
+      nb, na, c.L, tb, vs_b, vd_b, coul_b, xT, yT, nullptr);
   SBDX_CHECK_LAUNCH();
   mark(4);
   k_add_transposed<T, K><<<tblocks, 256, 0, st>>>(yT, y, na, nb);
@@ -705,13 +730,15 @@ void sigma_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
   const T* vd_b = f ? reinterpret_cast<const T*>(c.B.val_d32.get()) : reinterpret_cast<const T*>(c.B.val_d.get());
   const T* V = f ? reinterpret_cast<const T*>(c.eri4_32.get()) : reinterpret_cast<const T*>(c.eri4.get());
   const T* hii = f ? reinterpret_cast<const T*>(c.hii32.get()) : reinterpret_cast<const T*>(c.hii.get());
+  const T* ca = f ? reinterpret_cast<const T*>(c.coul_a32.get()) : reinterpret_cast<const T*>(c.coul_a.get());
+  const T* cb = f ? reinterpret_cast<const T*>(c.coul_b32.get()) : reinterpret_cast<const T*>(c.coul_b.get());
   auto go = [&](auto acc) {
     constexpr int A = decltype(acc)::value;
     switch (k) {
-      case 1: sigma_k<T, 1, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
-      case 2: sigma_k<T, 2, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
-      case 4: sigma_k<T, 4, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
-      case 8: sigma_k<T, 8, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii); break;
+      case 1: sigma_k<T, 1, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
+      case 2: sigma_k<T, 2, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
+      case 4: sigma_k<T, 4, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
+      case 8: sigma_k<T, 8, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
       default: throw Error(-1, "block width must be 1, 2, 4 or 8");
     }
   };
@@ -739,22 +766,30 @@ void reset_mix_tiles(sbdx_ctx& c) {
   for (auto& m : c.mix) m = MixTiles{};
 }
 
+// build (once) the mixed-pass tiles of the kernel variant a sigma build of precision prec and
+// block width k uses, so a later build does not allocate them
+void prepare_mix_tiles(sbdx_ctx& c, int prec, int k) {
+  const int var = mix_variant(c, prec, k);
+  const int threads = (var % 1000000) / 100;
+  (void)mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
+}
+
 // variants (1000000 (stages - 1) + 100 threads + entries per thread) of the mixed kernel for
 // block width K
 std::vector<int> mix_variant_candidates(int K) {
-  if (K == 1) return {25632, 25616, 51216, 102408, 1102408, 1051216};
+  if (K == 1) return {25632, 25616, 51216, 102408, 1102408, 3102408, 1051216, 1025616};
   if (K == 2) return {102404, 25608, 102408, 25616, 51216, 1102404, 3102404, 1051208};
   if (K == 4) return {25608, 25604, 102404, 1102404};
   return {25604, 51204};
 }
 
-// whether a variant's LDS (double-buffered V rows of its stages, reused by the fold's entry
-// groups of FP64 partial sums) fits a CU for this problem
+// whether a variant's LDS (double-buffered V rows of its stages, reused by the fold for one
+// vector's entry sums) fits a CU for this problem
 bool mix_variant_fits(const sbdx_ctx& c, int prec, int K, int var) {
-  const size_t stages = var / 1000000 + 1, es = prec == 0 ? 8 : 4;
+  (void)K;
+  const int stages = var / 1000000 + 1, es = prec == 0 ? 8 : 4;
   const int threads = (var % 1000000) / 100, ept = var % 100;
-  const size_t fold = static_cast<size_t>(fold_group(ept, threads, K, 8)) * threads * K * 8;
-  return std::max(2 * stages * c.L * c.L * es, fold) <= kMixMaxLds;
+  return mix_lds_bytes(c.L, c.A.max_s, stages, es, threads * ept, 8) <= kMixMaxLds;
 }
 
 void sigma_block(sbdx_ctx& c, int prec, int k, const void* x, void* y, hipStream_t st) {

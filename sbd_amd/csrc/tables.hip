@@ -236,6 +236,26 @@ void build_tables(sbdx_ctx& c, const uint64_t* alpha, const uint64_t* beta) {
   build_spin(c.B, beta, c.nb, c.L, c.stream);
 }
 
+// out[p * ncol + col] = sum over the occupied orbitals j of col_str[col], ascending, of
+// eri4[p L^2 + j L + j]
+__global__ __launch_bounds__(256) void k_coul_table(const uint64_t* __restrict__ col_str, int64_t ncol, int L,
+                                                    const double* __restrict__ eri4, double* __restrict__ out) {
+  const int64_t nt = cdiv(ncol, 256);
+  const int64_t p = blockIdx.x / nt;
+  const int64_t col = (blockIdx.x % nt) * 256 + threadIdx.x;
+  if (col >= ncol) return;
+  const int64_t L2 = static_cast<int64_t>(L) * L;
+  const double* __restrict__ vp = eri4 + p * L2;
+  double s = 0.0;
+  uint64_t o = col_str[col];
+  while (o) {
+    const int j = ctz64(o);
+    o &= o - 1;
+    s += vp[j * L + j];
+  }
+  out[p * ncol + col] = s;
+}
+
 void build_integral_tables(sbdx_ctx& c) {
   drop_stored(c);  // a stored H belongs to the previous integrals
   const int L = c.L;
@@ -275,9 +295,19 @@ void build_integral_tables(sbdx_ctx& c) {
       c.A.str.get(), c.B.str.get(), c.na, c.nb, L, c.I0, c.A.e1.get(), c.B.e1.get(), c.eri4.get(),
       c.hii.get(), nullptr);
   SBDX_CHECK_LAUNCH();
+  c.coul_a.alloc(L2 * c.nb);
+  c.coul_b.alloc(L2 * c.na);
+  c.coul_a32.release();
+  c.coul_b32.release();
+  k_coul_table<<<static_cast<unsigned>(L2 * cdiv(c.nb, 256)), 256, 0, st>>>(c.B.str.get(), c.nb, L, c.eri4.get(),
+                                                                             c.coul_a.get());
+  k_coul_table<<<static_cast<unsigned>(L2 * cdiv(c.na, 256)), 256, 0, st>>>(c.A.str.get(), c.na, L, c.eri4.get(),
+                                                                             c.coul_b.get());
+  SBDX_CHECK_LAUNCH();
   SBDX_HIP(hipStreamSynchronize(st));
   reset_mix_tiles(c);
   c.have_ints = true;
+  prepare_mix_tiles(c, SBDX_FP64, 1);  // the default single-vector FP64 build's tiles, at setup
 }
 
 void ensure_fp32(sbdx_ctx& c) {
@@ -296,6 +326,9 @@ void ensure_fp32(sbdx_ctx& c) {
   const int64_t L4 = static_cast<int64_t>(c.L) * c.L * c.L * c.L;
   conv(c.eri4, c.eri4_32, L4);
   conv(c.hii, c.hii32, c.ndet());
+  const int64_t L2 = static_cast<int64_t>(c.L) * c.L;
+  conv(c.coul_a, c.coul_a32, L2 * c.nb);
+  conv(c.coul_b, c.coul_b32, L2 * c.na);
   SBDX_HIP(hipStreamSynchronize(st));
 }
 

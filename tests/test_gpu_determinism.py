@@ -19,7 +19,7 @@ torch = pytest.importorskip("torch")
 
 # variants of the mixed kernel (1000000 (stages - 1) + 100 threads + entries per thread) per
 # block width; the same lists as the autotuner's candidates
-VARIANTS = {1: [25632, 25616, 51216, 102408, 1102408, 1051216],
+VARIANTS = {1: [25632, 25616, 51216, 102408, 1102408, 3102408, 1051216, 1025616],
             2: [102404, 25608, 102408, 25616, 51216, 1102404, 3102404, 1051208]}
 
 

@@ -92,7 +92,7 @@ def test_sigma_block_interleaved(pair, k, prec):
         assert rel <= tol, (v, rel)
 
 
-VARIANTS = {1: [25632, 25616, 51216, 102408, 1102408, 1051216],
+VARIANTS = {1: [25632, 25616, 51216, 102408, 1102408, 3102408, 1051216, 1025616],
             2: [102404, 25608, 102408, 25616, 51216, 1102404, 3102404, 1051208],
             4: [25608, 25604, 102404, 1102404], 8: [25604, 51204]}
 
@@ -186,6 +186,11 @@ def test_rdm(pair):
     one, two = eng.rdm(W)
     assert np.max(np.abs(one - one_r)) < 1e-12
     assert np.max(np.abs(two - two_r)) < 1e-12
+    # the device-vector entry point runs the same kernels on the same blocked vector
+    import torch
+    dW = torch.from_numpy(W).cuda()
+    one_d, two_d = eng.rdm_device(dW.data_ptr())
+    assert np.max(np.abs(one_d - one)) < 1e-14 and np.max(np.abs(two_d - two)) < 1e-14
     d = eng.occupation(W)
     assert np.max(np.abs(d - so.occupation(W))) < 1e-12
 
