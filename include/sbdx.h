@@ -87,6 +87,16 @@ int sbdx_sigma(sbdx_ctx *ctx, int nvec, const double *C, double *S);
 int sbdx_sigma_device(sbdx_ctx *ctx, int prec, int nvec, const void *dC, void *dS,
                       void *stream);
 
+/* No counterpart in the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): sbdx_sigma_device for one
+ * vector (nvec = 1), enqueued on `stream` (the context's stream if NULL) without allocating or
+ * synchronising.  Its scratch is a fixed per-device array of 64 MiB: ndet may be at most
+ * 2,097,152 in FP64 (4,194,304 in FP32), larger problems get SBDX_ERR_ARG.  Needs the tables of
+ * its precision built beforehand: FP64 after sbdx_set_integrals; FP32 after one sbdx_sigma_device
+ * call with SBDX_FP32 (SBDX_ERR_STATE otherwise).  Not with a stored Hamiltonian.  Calls that
+ * overlap on one device share the scratch and race. */
+int sbdx_sigma_device_noalloc(sbdx_ctx *ctx, int prec, const void *dC, void *dS, void *stream);
+
 /* Fused sigma of a block of k vectors (k = 1, 2, 4 or 8) stored interleaved: element
  * [det * k + v] of dC / dS (device pointers, type by prec).  One pass over the excitation lists
  * serves all k vectors (mult, chemistry/tpb/mult.h:387-648, applied to k vectors at once). */

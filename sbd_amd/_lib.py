@@ -24,7 +24,7 @@ EXPORTS = [
     "sbdx_mlp_train_defaults", "sbdx_mlp_record", "sbdx_mlp_loss_grad", "sbdx_mlp_forward",
     "sbdx_mlp_train", "sbdx_mlp_set", "sbdx_mlp_apply", "sbdx_autotune", "sbdx_set_mix_variant", "sbdx_set_fp32_accumulation",
     "sbdx_batch_create", "sbdx_batch_davidson", "sbdx_batch_destroy", "sbdx_store_hamiltonian",
-    "sbdx_bandwidth_probe", "sbdx_rdm_device",
+    "sbdx_bandwidth_probe", "sbdx_rdm_device", "sbdx_sigma_device_noalloc",
 ]
 MLP_NPARAM = 12609
 MLP_SAMPLE = np.dtype([("alpha", "<u8"), ("beta", "<u8"), ("xh", "<f4"), ("xr", "<f4"),
@@ -83,6 +83,7 @@ def lib():
     L.sbdx_sigma.argtypes = [vp, C.c_int, f64p, f64p]
     L.sbdx_sigma_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
     L.sbdx_sigma_block_device.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
+    L.sbdx_sigma_device_noalloc.argtypes = [vp, C.c_int, vp, vp, vp]
     L.sbdx_davidson_defaults.argtypes = [C.POINTER(DavidsonOpts)]
     L.sbdx_davidson_defaults.restype = None
     L.sbdx_davidson.argtypes = [vp, C.POINTER(DavidsonOpts), f64p, C.POINTER(DavidsonResult),
@@ -257,6 +258,11 @@ class Engine:
     def sigma_device(self, prec, nvec, dC, dS, stream=None):
         check(lib().sbdx_sigma_device(self._h, prec, nvec, C.c_void_p(dC), C.c_void_p(dS),
                                       C.c_void_p(stream) if stream else None))
+
+    def sigma_device_noalloc(self, prec, dC, dS, stream=None):
+        """sbdx_sigma_device_noalloc: one vector, no allocation, enqueued on `stream`."""
+        check(lib().sbdx_sigma_device_noalloc(self._h, prec, C.c_void_p(dC), C.c_void_p(dS),
+                                              C.c_void_p(stream) if stream else None))
 
     def sigma_block_device(self, prec, k, dC, dS, stream=None):
         """Fused sigma of k interleaved vectors (element [det * k + v])."""

@@ -28,6 +28,7 @@ void mlp_forward(const sbdx_mlp_sample* S, int64_t n, const float* params, float
 void mlp_train(const sbdx_mlp_sample* S, int64_t n, const sbdx_mlp_train_opts& o, float* params, double* trace,
                int trace_cap);
 double bandwidth_probe(int kind, int64_t bytes, int reps);
+void sigma_noalloc(sbdx_ctx& c, int prec, const void* dC, void* dS, hipStream_t st);
 }  // namespace sbdx
 
 namespace {
@@ -102,6 +103,7 @@ int sbdx_create(int norb, int64_t n_alpha, const uint64_t* alpha, int64_t n_beta
     auto* c = new sbdx_ctx();
     try {
       SBDX_HIP(hipGetDevice(&c->device));
+      (void)sbdx::noalloc_scratch(c->device);  // once per device: the no-allocation sigma's scratch
       SBDX_HIP(hipStreamCreate(&c->stream));
       c->L = norb;
       c->na = n_alpha;
@@ -198,6 +200,14 @@ int sbdx_sigma_device(sbdx_ctx* c, int prec, int nvec, const void* dC, void* dS,
       sbdx::apply_sign(*c, prec, 1, c->ws.tmp_out.get(), yout, st);
     }
     SBDX_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int sbdx_sigma_device_noalloc(sbdx_ctx* c, int prec, const void* dC, void* dS, void* stream) {
+  return guarded([&] {
+    need_ints(c);
+    need(dC && dS && (prec == SBDX_FP64 || prec == SBDX_FP32), "bad argument");
+    sbdx::sigma_noalloc(*c, prec, dC, dS, stream ? static_cast<hipStream_t>(stream) : c->stream);
   });
 }
 

@@ -141,6 +141,11 @@ struct sbdx_ctx {
   int mix_variant_tuned[2][4] = {}; // autotuned mixed-kernel variant [FP64|FP32][K slot]
   int fp32_acc = SBDX_ACC_FP64;     // accumulation of FP32 sigma builds (sbdx_set_fp32_accumulation)
   bool stored = false;             // H stored as CSR (stored.hip): sigma builds use it
+  // set while sbdx_sigma_device_noalloc runs: the sigma path then takes its transposed pair
+  // from noalloc_xT/yT and treats any allocation it would make as an error
+  bool no_alloc = false;
+  void* noalloc_xT = nullptr;
+  void* noalloc_yT = nullptr;
   sbdx::DevBuf<int64_t> h_rowptr;
   sbdx::DevBuf<uint32_t> h_col;
   sbdx::DevBuf<double> h_val;
@@ -231,6 +236,7 @@ void build_integral_tables(sbdx_ctx& c);        // tables.hip: values, H_ii, S
 void ensure_fp32(sbdx_ctx& c);                  // tables.hip: float copies
 void reset_mix_tiles(sbdx_ctx& c);              // sigma.hip: drop cached mixed-pass tiles
 void prepare_mix_tiles(sbdx_ctx& c, int prec, int k);  // sigma.hip: build the default variant's tiles
+void* noalloc_scratch(int device);              // noalloc.hip: device address of the scratch array
 std::vector<int> mix_variant_candidates(int k); // sigma.hip: autotuner search space
 bool mix_variant_fits(const sbdx_ctx& c, int prec, int k, int variant);  // its LDS fits a CU
 // sigma of nvec vectors stored one after another (vector-major)

@@ -511,6 +511,9 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
     if (m.cap == cap && m.ent_off.get()) return m;
     if (!slot && m.cap == 0) slot = &m;
   }
+  if (c.no_alloc)
+    throw Error(SBDX_ERR_STATE, "the mixed-pass tiles of this precision's kernel variant are not built: run "
+                                "sbdx_sigma_device once with this precision first");
   // tile sets are never evicted: captured graphs (batches) hold their addresses.  The
   // variants use four tile sizes (1024 to 8192 entries), fewer than the slots.
   if (!slot) throw Error(SBDX_ERR_STATE, "no free slot for another mixed-pass tile size");
@@ -693,10 +696,12 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   launch_mixed<T, K, kAcc>(c, x, y, V, st);
   mark(2);
   // beta rows on the transposed vector
-  c.ws.xT.ensure(n * K * sizeof(T));
-  c.ws.yT.ensure(n * K * sizeof(T));
-  T* xT = reinterpret_cast<T*>(c.ws.xT.get());
-  T* yT = reinterpret_cast<T*>(c.ws.yT.get());
+  if (!c.no_alloc) {
+    c.ws.xT.ensure(n * K * sizeof(T));
+    c.ws.yT.ensure(n * K * sizeof(T));
+  }
+  T* xT = reinterpret_cast<T*>(c.no_alloc ? c.noalloc_xT : c.ws.xT.get());
+  T* yT = reinterpret_cast<T*>(c.no_alloc ? c.noalloc_yT : c.ws.yT.get());
   constexpr int S = K * sizeof(T) <= 8 ? 64 : (K * sizeof(T) <= 32 ? 32 : 16);
   const unsigned tblocks = static_cast<unsigned>(cdiv(na, S) * cdiv(nb, S));
   k_transpose<T, K><<<tblocks, 256, 0, st>>>(x, xT, na, nb);

@@ -307,11 +307,13 @@ void build_integral_tables(sbdx_ctx& c) {
   SBDX_HIP(hipStreamSynchronize(st));
   reset_mix_tiles(c);
   c.have_ints = true;
-  prepare_mix_tiles(c, SBDX_FP64, 1);  // the default single-vector FP64 build's tiles, at setup
+  prepare_mix_tiles(c, SBDX_FP64, 1);  // FP64 single-vector builds allocate nothing more
 }
 
 void ensure_fp32(sbdx_ctx& c) {
   if (c.hii32.get()) return;
+  if (c.no_alloc)
+    throw Error(SBDX_ERR_STATE, "the FP32 tables are not built: run sbdx_sigma_device once with SBDX_FP32 first");
   hipStream_t st = c.stream;
   const int bs = 256;
   auto conv = [&](const DevBuf<double>& x, DevBuf<float>& y, int64_t n) {
