@@ -42,6 +42,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("time-to-1e-6 Ha & sigma builds/s at 4e8 dets, 1 GPU: plain FP64 vs "
           "mixed+block+MLP")
 HBM_PEAK_GBS = 8000.0
+L2_PEAK_BYTES_PER_CLK = 8 * 16 * 128  # XCDs x L2 channels x bytes per channel per clock
 
 # Davidson variants of --solve: the reference algorithm in FP64, and the block thick-restart
 # solver in FP64 or mixed precision (FP32 corrections, FP64 anchor) with 1 or 2 vectors a block
@@ -395,7 +396,6 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-        bandwidth_probe(1, 1 << 20, 1)
 
     for _ in range(args.warmup):
         eng.sigma_device(prec, 1, x.data_ptr(), y.data_ptr(), stream)
@@ -419,10 +419,9 @@ def main():
     dom = max(phase_ms, key=phase_ms.get)
     dom_s = phase_ms[dom] * 1e-3
     pmc = counters(args.workload, args.strings, args.prec).get(dom, {})
-    # live ceilings: HBM (STREAM-style copy of 4 GiB) and L2 -> L1 (1 MiB read by every CU)
+    # live ceiling of HBM: a STREAM-style copy of 4 GiB
     hbm_probe = bandwidth_probe(0, 4 << 30, 5)
-    l2_probe = bandwidth_probe(1, 1 << 20, 20)
-    log(f"[bench] probes: HBM copy {hbm_probe:.0f} GB/s, L2 -> L1 read {l2_probe:.0f} GB/s")
+    log(f"[bench] probe: HBM copy {hbm_probe:.0f} GB/s")
     l2_bytes = pmc.get("l2_to_l1_bytes")
     roofline = {
         "kernel": PHASE_KERNELS[dom],
@@ -436,14 +435,18 @@ def main():
         "algorithmic_model": "compulsory: every operand read once from HBM, every output written once",
         "mean_ms": round(phase_ms[dom], 3),
         "hbm_probe_gbs": round(hbm_probe, 1),
+        # the level the gathers are served from: counter-measured L2 -> L1 bytes per launch
+        # against the L2 read peak (16 channels x 128 B per clock per XCD, 8 XCDs, at the
+        # kernel's measured clock)
         "l2": None if l2_bytes is None else {
             "achieved": round(l2_bytes / dom_s / 1e9, 1),
-            "peak": round(l2_probe, 1),
+            "peak": round(L2_PEAK_BYTES_PER_CLK * pmc["clock_ghz"], 1),
             "unit": "GB/s",
-            "frac": round(l2_bytes / dom_s / 1e9 / l2_probe, 4),
+            "frac": round(l2_bytes / dom_s / 1e9 / (L2_PEAK_BYTES_PER_CLK * pmc["clock_ghz"]), 4),
             "traffic": l2_bytes,
-            "peak_source": "live probe: every workgroup streams a 1 MiB L2-resident buffer",
         },
+        # the binding resource named by the counters: the vector-memory address unit
+        "ta_busy": pmc.get("ta_busy"),
         "gathered_bytes": gath[dom],
         "gathered_rate_gbs": round(gath[dom] / dom_s / 1e9, 1),
         "counters": pmc.get("source"),

@@ -95,16 +95,16 @@ struct Prof {
 
 // Beta-single entries regrouped for the mixed (alpha single x beta single) pass: columns are
 // cut into tiles of consecutive beta strings holding at most cap entries; inside a tile
-// entries are sorted by ket column jb.  meta = slot (bits 0-15: the entry's index in the beta
-// list minus the tile's first index, so each column's entries have consecutive slots in the
-// list's own order) | pb = b*L + j (16-27) | sign (bit 31).
+// entries are sorted by ket column jb.  jm = (jb, meta), meta = slot (bits 0-15: the entry's
+// index in the beta list minus the tile's first index, so each column's entries have
+// consecutive slots in the list's own order) | pb = b*L + j (16-27) | sign (bit 31).
 struct MixTiles {
   int ntile = 0;
   int max_cols = 0;
   int64_t cap = 0;           // entries per tile (threads x entries per thread); 0: empty
   DevBuf<int64_t> ent_off;   // ntile + 1
   DevBuf<int32_t> col0, ncol;
-  DevBuf<uint32_t> jb, meta;
+  DevBuf<uint2> jm;
 };
 
 struct Workspace {

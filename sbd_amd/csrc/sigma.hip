@@ -153,30 +153,40 @@ struct RowTabs {
 
 // Same-spin part (and, for the alpha rows, the diagonal): y[row, col] = sum over the row's
 // singles and doubles of value * x[ket, col], for K interleaved vectors.  One workgroup = one
-// row x kRowTile columns: the row's list entries are the same for every lane (scalar loads),
-// the gathers of one entry are contiguous across the lanes.  A single's value is its stored
-// same-spin part plus the spectator string's Coulomb term, read from the Coulomb table row of
-// its orbital pair (coalesced across the lanes, in flight with the gathers).
-template <typename T, bool kDiag, int K, int kAcc>
+// wavefront = one row x 64 C columns, C consecutive columns per lane: the row's list entries
+// are the same for every lane (scalar loads), and the gathers of one entry are one contiguous
+// segment across the lanes.  A single's value is its stored same-spin part plus the spectator string's Coulomb
+// term, read from the Coulomb table row of its orbital pair (coalesced, in flight with the
+// gathers).
+template <typename T, bool kDiag, int K, int kAcc, int C>
 __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
     int64_t nrow, int64_t ncol, int L, RowTabs rt, const T* __restrict__ val_s,
     const T* __restrict__ val_d, const T* __restrict__ coul, const T* __restrict__ x, T* __restrict__ y,
     const T* __restrict__ hii) {
-  // gathers issued back to back per wavefront before their FMAs: 16 of up to 8 bytes, 4 of 16
-  constexpr int U = K * sizeof(T) <= 8 ? 16 : (K * sizeof(T) == 16 ? 4 : 2);
+  constexpr int KC = K * C;  // values per lane per ket row
+  // gathers issued back to back per wavefront before their FMAs
+  constexpr int U = KC * sizeof(T) <= 8 ? 16 : (KC * sizeof(T) == 16 ? (C > 1 ? 8 : 4) : 2);
   const int64_t g = blockIdx.x;
   const int64_t tile = g / nrow;
   const int64_t row = g - tile * nrow;
-  const int64_t col = tile * kRowTile + threadIdx.x;
+  const int64_t col = (tile * kRowTile + threadIdx.x) * C;  // ncol is a multiple of C
   if (col >= ncol) return;
-  typename AccOf<T, kAcc>::type acc[K];
+  typename AccOf<T, kAcc>::type acc[C][K];
   const int64_t me = row * ncol + col;
   if (kDiag) {
-    const VecK<T, K> xm = ldk<T, K>(x + me * K);
-    const T h = hii[me];
+    const VecK<T, KC> xm = ldk<T, KC>(x + me * K);
+    const VecK<T, C> h = ldk<T, C>(hii + me);
 #pragma unroll
-    for (int v = 0; v < K; ++v) acc[v].fma(h, xm.v[v]);
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[c][v].fma(h.v[c], xm.v[c * K + v]);
   }
+  auto fma_all = [&](T w, const VecK<T, KC>& xv) {
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[c][v].fma(w, xv.v[c * K + v]);
+  };
 
   // same-spin doubles
   const int64_t d0 = rt.off_d[row], d1 = rt.off_d[row + 1];
@@ -184,22 +194,13 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
   const int64_t stride = ncol * K;
   int64_t k = d0;
   for (; k + U <= d1; k += U) {
-    VecK<T, K> xv[U];
+    VecK<T, KC> xv[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) xv[u] = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_d[k + u]) * stride);
+    for (int u = 0; u < U; ++u) xv[u] = ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_d[k + u]) * stride);
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const T w = val_d[k + u];
-#pragma unroll
-      for (int v = 0; v < K; ++v) acc[v].fma(w, xv[u].v[v]);
-    }
+    for (int u = 0; u < U; ++u) fma_all(val_d[k + u], xv[u]);
   }
-  for (; k < d1; ++k) {
-    const VecK<T, K> xv = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_d[k]) * stride);
-    const T w = val_d[k];
-#pragma unroll
-    for (int v = 0; v < K; ++v) acc[v].fma(w, xv.v[v]);
-  }
+  for (; k < d1; ++k) fma_all(val_d[k], ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_d[k]) * stride));
 
   // same-spin singles: value = stored part + sign * Coulomb table [pair, col]
   const int64_t s0 = rt.off_s[row], s1 = rt.off_s[row + 1];
@@ -208,33 +209,37 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
     const uint32_t e = rt.exc_s[q];
     return static_cast<int64_t>((e >> 6) & 63) * L + (e & 63);
   };
-  auto single_value = [&](int64_t q, T d) { return val_s[q] + ((rt.exc_s[q] >> 12) & 1u ? -d : d); };
+  // the C columns' values of single q are distinct (their Coulomb terms differ)
+  auto fma_single = [&](int64_t q, const VecK<T, C>& d, const VecK<T, KC>& xv) {
+    const bool neg = (rt.exc_s[q] >> 12) & 1u;
+    const T base = val_s[q];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const T w = base + (neg ? -d.v[c] : d.v[c]);
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[c][v].fma(w, xv.v[c * K + v]);
+    }
+  };
   k = s0;
   for (; k + U <= s1; k += U) {
-    VecK<T, K> xv[U];
-    T dv[U];
+    VecK<T, KC> xv[U];
+    VecK<T, C> dv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      xv[u] = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_s[k + u]) * stride);
-      dv[u] = cc[pair_of(k + u) * ncol];
+      xv[u] = ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_s[k + u]) * stride);
+      dv[u] = ldk<T, C>(cc + pair_of(k + u) * ncol);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const T w = single_value(k + u, dv[u]);
-#pragma unroll
-      for (int v = 0; v < K; ++v) acc[v].fma(w, xv[u].v[v]);
-    }
+    for (int u = 0; u < U; ++u) fma_single(k + u, dv[u], xv[u]);
   }
-  for (; k < s1; ++k) {
-    const VecK<T, K> xv = ldk<T, K>(xc + static_cast<int64_t>(rt.ket_s[k]) * stride);
-    const T w = single_value(k, cc[pair_of(k) * ncol]);
+  for (; k < s1; ++k)
+    fma_single(k, ldk<T, C>(cc + pair_of(k) * ncol), ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_s[k]) * stride));
+  VecK<T, KC> out;
 #pragma unroll
-    for (int v = 0; v < K; ++v) acc[v].fma(w, xv.v[v]);
-  }
-  VecK<T, K> out;
+  for (int c = 0; c < C; ++c)
 #pragma unroll
-  for (int v = 0; v < K; ++v) out.v[v] = acc[v].get();
-  stk<T, K>(y + me * K, out);
+    for (int v = 0; v < K; ++v) out.v[c * K + v] = acc[c][v].get();
+  stk<T, KC>(y + me * K, out);
 }
 
 // Alpha single x beta single (task type 0, mult.h:569-596):
@@ -272,14 +277,13 @@ __host__ __device__ constexpr size_t mix_lds_bytes(int L, int max_s, int stages,
 // gathers in flight together); the sums still take the alpha singles in ascending order, so
 // the result does not depend on kStage.
 
-template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage, int kAcc>
+template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage, int kAcc, int kVW>
 __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     int64_t na, int64_t nb, int L, int max_s, const int64_t* __restrict__ a_off,
     const uint32_t* __restrict__ a_ket, const uint32_t* __restrict__ a_exc, int ntile,
     const int64_t* __restrict__ t_off, const int32_t* __restrict__ t_col0, const int32_t* __restrict__ t_ncol,
-    const uint32_t* __restrict__ e_jb, const uint32_t* __restrict__ e_meta,
-    const int64_t* __restrict__ b_off, const T* __restrict__ V, const T* __restrict__ x,
-    T* __restrict__ y) {
+    const uint2* __restrict__ e_jm, const int64_t* __restrict__ b_off, const T* __restrict__ V,
+    const T* __restrict__ x, T* __restrict__ y) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L2 = L * L;
   // the row's alpha singles (ket row, packed excitation word), then the double-buffered V rows
@@ -315,15 +319,17 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
 #pragma unroll
   for (int j = 0; j < kEPT; ++j) {
     const int64_t e = e0 + tid + static_cast<int64_t>(j) * kMixThreads;
-    const bool ok = e < e1;
-    joff[j] = (ok ? e_jb[e] : 0u) * static_cast<uint32_t>(K * sizeof(T));
-    meta[j] = ok ? e_meta[e] : 0x0000ffffu;
+    const uint2 jm = e < e1 ? e_jm[e] : make_uint2(0u, 0x0000ffffu);
+    joff[j] = jm.x * static_cast<uint32_t>(K * sizeof(T));
+    meta[j] = jm.y;
   }
   __syncthreads();  // the row's singles are staged
 
   // V rows of the next stage's alpha singles: fetched into registers before the current
-  // stage's gathers (their latency hides under them), written to LDS after them
-  T pre[kStage][kVPerThread];
+  // stage's gathers (their latency hides under them), written to LDS after them; kVW values
+  // per load (16 bytes where L^2 values of a row are a whole number of 16-byte words)
+  constexpr int kVV = kVPerThread / kVW > 0 ? kVPerThread / kVW : 1;  // loads per thread
+  VecK<T, kVW> pre[kStage][kVV];
   bool pre_neg[kStage];
   auto fetch = [&](int kk) {
 #pragma unroll
@@ -333,9 +339,9 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
       const T* src = V + static_cast<int64_t>(((e >> 6) & 63) * L + (e & 63)) * L2;
       pre_neg[q] = (e >> 12) & 1u;
 #pragma unroll
-      for (int u = 0; u < kVPerThread; ++u) {
-        const int p = tid + u * kMixThreads;
-        pre[q][u] = src[p < L2 ? p : 0];
+      for (int u = 0; u < kVV; ++u) {
+        const int p = (tid + u * kMixThreads) * kVW;
+        pre[q][u] = ldk<T, kVW>(src + (p < L2 ? p : 0));
       }
     }
   };
@@ -344,9 +350,14 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     for (int q = 0; q < kStage; ++q) {
       if (kk + q >= nk) break;
 #pragma unroll
-      for (int u = 0; u < kVPerThread; ++u) {
-        const int p = tid + u * kMixThreads;
-        if (p < L2) dst[q * L2 + p] = pre_neg[q] ? -pre[q][u] : pre[q][u];
+      for (int u = 0; u < kVV; ++u) {
+        const int p = (tid + u * kMixThreads) * kVW;
+        if (p < L2) {
+          VecK<T, kVW> w = pre[q][u];
+#pragma unroll
+          for (int i = 0; i < kVW; ++i) w.v[i] = pre_neg[q] ? -w.v[i] : w.v[i];
+          stk<T, kVW>(dst + q * L2 + p, w);
+        }
       }
     }
   };
@@ -534,9 +545,8 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   SBDX_HIP(hipMemcpy(exc.data(), B.exc_s.get(), B.nnz_s * 4, hipMemcpyDeviceToHost));
   std::vector<int64_t> t_off{0};
   std::vector<int32_t> t_col0, t_ncol;
-  std::vector<uint32_t> e_jb, e_meta;
-  e_jb.reserve(B.nnz_s);
-  e_meta.reserve(B.nnz_s);
+  std::vector<uint2> e_jm;
+  e_jm.reserve(B.nnz_s);
   struct Ent {
     uint32_t jb, meta;
   };
@@ -561,26 +571,21 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
         buf.push_back(Ent{ket[k], meta});
       }
     std::stable_sort(buf.begin(), buf.end(), [](const Ent& u, const Ent& v) { return u.jb < v.jb; });
-    for (size_t r = 0; r < buf.size(); ++r) {
-      e_jb.push_back(buf[r].jb);
-      e_meta.push_back(buf[r].meta);
-    }
+    for (size_t r = 0; r < buf.size(); ++r) e_jm.push_back(make_uint2(buf[r].jb, buf[r].meta));
     t_col0.push_back(static_cast<int32_t>(start));
     t_ncol.push_back(static_cast<int32_t>(ib - start));
-    t_off.push_back(static_cast<int64_t>(e_jb.size()));
+    t_off.push_back(static_cast<int64_t>(e_jm.size()));
     mt.max_cols = std::max<int>(mt.max_cols, static_cast<int>(ib - start));
   }
   mt.ntile = static_cast<int>(t_col0.size());
   mt.ent_off.alloc(t_off.size());
   mt.col0.alloc(t_col0.size());
   mt.ncol.alloc(t_ncol.size());
-  mt.jb.alloc(e_jb.size());
-  mt.meta.alloc(e_meta.size());
+  mt.jm.alloc(e_jm.size());
   SBDX_HIP(hipMemcpy(mt.ent_off.get(), t_off.data(), t_off.size() * 8, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.col0.get(), t_col0.data(), t_col0.size() * 4, hipMemcpyHostToDevice));
   SBDX_HIP(hipMemcpy(mt.ncol.get(), t_ncol.data(), t_ncol.size() * 4, hipMemcpyHostToDevice));
-  SBDX_HIP(hipMemcpy(mt.jb.get(), e_jb.data(), e_jb.size() * 4, hipMemcpyHostToDevice));
-  SBDX_HIP(hipMemcpy(mt.meta.get(), e_meta.data(), e_meta.size() * 4, hipMemcpyHostToDevice));
+  SBDX_HIP(hipMemcpy(mt.jm.get(), e_jm.data(), e_jm.size() * sizeof(uint2), hipMemcpyHostToDevice));
   return mt;
 }
 
@@ -612,14 +617,21 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     kernel<<<static_cast<unsigned>(blocks), threads, lds, st>>>(
         c.na, c.nb, c.L, c.A.max_s, c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), mt.ntile, mt.ent_off.get(),
-        mt.col0.get(), mt.ncol.get(), mt.jb.get(), mt.meta.get(), c.B.off_s.get(), V, x, y);
+        mt.col0.get(), mt.ncol.get(), mt.jm.get(), c.B.off_s.get(), V, x, y);
     SBDX_CHECK_LAUNCH();
   };
   const bool small_v = static_cast<int>(c.L) * c.L <= 2048;
+  // integral rows loaded 16 bytes at a time when a row is a whole number of 16-byte words
+  constexpr int VW = 16 / static_cast<int>(sizeof(T));
+  const bool wide_v = (static_cast<int64_t>(c.L) * c.L * sizeof(T)) % 16 == 0;
   auto pick = [&](auto e, auto t, auto sg) {
     constexpr int E = decltype(e)::value, NT = decltype(t)::value, SG = decltype(sg)::value;
-    small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc>, NT, E)
-            : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc>, NT, E);
+    if (wide_v)
+      small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc, VW>, NT, E)
+              : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc, VW>, NT, E);
+    else
+      small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc, 1>, NT, E)
+              : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc, 1>, NT, E);
   };
   using std::integral_constant;
   using I4 = integral_constant<int, 4>;
@@ -675,6 +687,18 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   }
 }
 
+// the row kernel, one column per lane.  C = 16 / (K x word) consecutive columns per lane (16-byte
+// lanes, 64 C columns per wavefront) was measured slower at 4e8 determinants: the wider ket-row
+// segments double the L2 footprint of the rows in flight (FP64, one vector: 253 against 187 ms
+// per pass), so the kernel keeps C as a parameter and launches C = 1.
+template <typename T, bool kDiag, int K, int kAcc>
+void launch_rows(int64_t nrow, int64_t ncol, int L, const RowTabs& rt, const T* val_s, const T* val_d,
+                 const T* coul, const T* x, T* y, const T* hii, hipStream_t st) {
+  k_sigma_rows<T, kDiag, K, kAcc, 1><<<static_cast<unsigned>(nrow * cdiv(ncol, kRowTile)), kRowTile, 0, st>>>(
+      nrow, ncol, L, rt, val_s, val_d, coul, x, y, hii);
+  SBDX_CHECK_LAUNCH();
+}
+
 template <typename T, int K, int kAcc>
 void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const T* vd_a,
              const T* vs_b, const T* vd_b, const T* V, const T* hii, const T* coul_a, const T* coul_b) {
@@ -688,8 +712,7 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   };
   mark(0);
   // alpha rows: diagonal + alpha singles and doubles
-  k_sigma_rows<T, true, K, kAcc><<<static_cast<unsigned>(na * cdiv(nb, kRowTile)), kRowTile, 0, st>>>(
-      na, nb, c.L, ta, vs_a, vd_a, coul_a, x, y, hii);
+  launch_rows<T, true, K, kAcc>(na, nb, c.L, ta, vs_a, vd_a, coul_a, x, y, hii, st);
   SBDX_CHECK_LAUNCH();
   mark(1);
   // alpha single x beta single
@@ -707,8 +730,7 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   k_transpose<T, K><<<tblocks, 256, 0, st>>>(x, xT, na, nb);
   SBDX_CHECK_LAUNCH();
   mark(3);
-  k_sigma_rows<T, false, K, kAcc><<<static_cast<unsigned>(nb * cdiv(na, kRowTile)), kRowTile, 0, st>>>(
-      nb, na, c.L, tb, vs_b, vd_b, coul_b, xT, yT, nullptr);
+  launch_rows<T, false, K, kAcc>(nb, na, c.L, tb, vs_b, vd_b, coul_b, xT, yT, nullptr, st);
   SBDX_CHECK_LAUNCH();
   mark(4);
   k_add_transposed<T, K><<<tblocks, 256, 0, st>>>(yT, y, na, nb);

@@ -131,11 +131,70 @@ def test_zero_last_layer_is_the_diagonal_preconditioner():
     assert abs(a["steps"] - b["steps"]) <= 6
 
 
+def c5_case():
+    """The learned preconditioner's workload family at test size: 48 orbitals, 12 + 12
+    electrons, rank-96 integrals, 200 x 220 sampled strings (44,000 determinants)."""
+    inst = synth.make_custom(48, 12, 12, 96, 200, 220, seed_ints=3)
+    return inst, O.System.from_instance(inst), Engine.from_instance(inst)
+
+
+def test_c5_family_record_train_apply_solve():
+    """Every GPU part of the learned preconditioner on the 48-orbital family against the
+    oracle: the recorder's pairs (features, diagonal and exact corrections, weights), the loss
+    and gradient on those pairs, training on the GPU, the trained network's correction, and
+    the mixed-precision solver with it reaching the FP64 energy."""
+    print("[mlp c5 family 200x220]", flush=True)
+    inst, so, eng = c5_case()
+    hii = so.hdiag()
+    xs = so.davidson(max_it=300, num_block=10, eps=1e-10)["W"]
+    # an iterate 0.96 along x*: on this family the weights concentrate on few determinants
+    # near x*, so the recorder keeps ~1,000 of 44,000 at this target
+    x = xs + 0.3 * np.random.default_rng(11).standard_normal(so.ndet) / np.sqrt(so.ndet)
+    x /= np.linalg.norm(x)
+    dx = torch.tensor(x, device="cuda")
+    dxs = torch.tensor(xs, device="cuda")
+    got, cnt, theta = eng.mlp_record(dx.data_ptr(), dxs.data_ptr(), 30000, seed=21)
+    ref, _, rtheta, _ = M.record(lambda v: so.mult(v, hii), hii, inst.alpha, inst.beta, x, xs, 30000, 21)
+    assert abs(theta - rtheta) < 1e-12 * abs(rtheta) and cnt == got.size and got.size > 300
+    key = lambda s: s["alpha"].astype(object) * (1 << 64) + s["beta"].astype(object)
+    gk = {k: i for i, k in enumerate(key(got))}
+    rk = {k: i for i, k in enumerate(key(ref))}
+    common = sorted(set(gk) & set(rk))
+    assert len(common) >= 0.99 * max(len(gk), len(rk))
+    g, r = got[[gk[k] for k in common]], ref[[rk[k] for k in common]]
+    for f in ("xh", "xr", "u", "v"):
+        assert np.max(np.abs(g[f] - r[f])) <= 1e-5 * max(1.0, np.max(np.abs(r[f]))), f
+    # loss and gradient of a non-trivial network on the recorded pairs
+    p = params(7)
+    loss, grad = mlp_loss_grad(got, p)
+    rl, rg = M.loss_grad(p, got)
+    assert abs(loss - rl) <= 1e-5 * abs(rl)
+    for name, (a, b) in GROUPS.items():
+        assert np.linalg.norm(grad[a:b] - rg[a:b]) <= 1e-4 * max(1e-30, np.linalg.norm(rg[a:b])), name
+    # training from the diagonal preconditioner (zero last layer) lowers the loss
+    p0 = mlp_train.init_params(0)
+    l0, _ = mlp_loss_grad(got, p0)
+    pt, trace = train(got, p0, epochs=40, batch=256, lr=2e-3, seed=3)
+    l1, _ = mlp_loss_grad(got, pt)
+    assert np.all(np.isfinite(trace)) and l1 < l0, (l0, l1)
+    # the trained network's correction, then the solver with it
+    eng.mlp_set(pt)
+    rr = np.random.default_rng(12).standard_normal(so.ndet)
+    th = hii.min() - 0.3
+    assert np.max(np.abs(eng.mlp_apply(th, rr) - M.precondition(pt, inst.alpha, inst.beta, hii, th, rr))) \
+        <= 1e-5 * np.max(np.abs(rr / (th - hii)))
+    # from the HF vector the one-vector FP64 solver settles on an excited state of this instance;
+    # the FP64 reference is the two-vector block solver, which reaches the lowest one
+    e64 = eng.davidson(method=1, precision=0, block=2, num_block=8, max_iteration=400, eps=1e-9)["energy"]
+    res = eng.davidson(method=1, precision=1, block=2, num_block=8, max_iteration=400, eps=1e-9,
+                       preconditioner=1)
+    assert res["converged"] and abs(res["energy"] - e64) < 1e-9, (res["energy"], e64)
+
+
 def test_training_lowers_the_loss_and_the_solver_stays_exact():
     """Pairs recorded from two 36-orbital instances train the network on the GPU; the loss
     drops below the untrained (diagonal) value, and on a held-out instance the learned
-    preconditioner converges to the same FP64 energy (1e-9 Ha).  The steps to 1e-6 Ha with and
-    without it are printed: that is the gain (or loss) of the trained network."""
+    preconditioner converges to the same FP64 energy (1e-9 Ha)."""
     print("[mlp training]", flush=True)
     pairs = []
     for sd in (100, 101):
@@ -152,10 +211,30 @@ def test_training_lowers_the_loss_and_the_solver_stays_exact():
     eng = Engine.from_instance(mlp_train.family_instance("c2", 150, 200))
     x = torch.zeros(eng.ndet, dtype=torch.float64, device="cuda")
     ref = mlp_train.solve(eng, x.data_ptr(), eps=1e-9)
-    ev = mlp_train.evaluate(eng, p, ref["energy"])
     eng.mlp_set(p)
     r = mlp_train.solve(eng, x.data_ptr(), preconditioner=1, eps=1e-9)
-    print(f"loss {l0:.4e} -> {l1:.4e}; steps to 1e-6 Ha: diagonal {ev[0]['steps_to_tol']}, "
-          f"learned {ev[1]['steps_to_tol']} (fallback at {ev[1]['fallback']})", flush=True)
     assert r["converged"] and abs(r["energy"] - ref["energy"]) < 1e-9
-    assert ev[0]["steps_to_tol"] > 0 and ev[1]["steps_to_tol"] > 0
+
+
+@pytest.mark.xfail(strict=False, reason=(
+    "expected negative: on these rank-72 random-integral instances a per-determinant scaling "
+    "of the diagonal correction is a weak lever (the least-squares scalings that best match the "
+    "exact corrections are near zero for most determinant classes), so the network trained on "
+    "two small instances does not reliably take fewer steps than the diagonal preconditioner; "
+    "the measured gains and losses at scale are recorded with the workload's profiles"))
+def test_learned_preconditioner_takes_fewer_steps_on_a_held_out_instance():
+    """Steps to 1e-6 Ha on a held-out 36-orbital instance with the diagonal and with the trained
+    learned preconditioner (network trained on pairs from two other seeds); the gain claimed is
+    fewer steps."""
+    print("[mlp gain]", flush=True)
+    pairs = [mlp_train.collect(Engine.from_instance(mlp_train.family_instance("c2", 150, sd)),
+                               20000, sd)[0] for sd in (100, 101)]
+    p, _ = train(np.concatenate(pairs), mlp_train.init_params(0), epochs=15, batch=4096, lr=2e-3,
+                 seed=1)
+    eng = Engine.from_instance(mlp_train.family_instance("c2", 150, 200))
+    x = torch.zeros(eng.ndet, dtype=torch.float64, device="cuda")
+    ref = mlp_train.solve(eng, x.data_ptr(), eps=1e-9)
+    ev = mlp_train.evaluate(eng, p, ref["energy"])
+    print(f"steps to 1e-6 Ha: diagonal {ev[0]['steps_to_tol']}, learned {ev[1]['steps_to_tol']} "
+          f"(fallback at {ev[1]['fallback']})", flush=True)
+    assert 0 < ev[1]["steps_to_tol"] < ev[0]["steps_to_tol"]
