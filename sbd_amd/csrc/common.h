@@ -81,6 +81,13 @@ struct SpinTab {
   DevBuf<double> e1;             // same-spin diagonal energy of each string
   int64_t nnz_s = 0, nnz_d = 0;
   int max_s = 0, max_d = 0;      // longest row of each list
+  // the lists cut into nblk ket blocks of ceil(n / nblk) kets: offsets in (row, block) order,
+  // entry row * nblk + b starts block b of the row (n * nblk + 1 entries), built on first use
+  struct KetBlocks {
+    int nblk = 0;
+    DevBuf<int64_t> s, d;
+  };
+  KetBlocks kblk[4];
 };
 
 // Per-phase timing of sigma builds (HIP events on the launch stream), for measurement only.

@@ -160,7 +160,7 @@ struct RowTabs {
 // gathers).
 template <typename T, bool kDiag, int K, int kAcc, int C>
 __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
-    int64_t nrow, int64_t ncol, int L, RowTabs rt, const T* __restrict__ val_s,
+    int64_t nrow, int64_t ncol, int L, RowTabs rt, int nblk, int blk, const T* __restrict__ val_s,
     const T* __restrict__ val_d, const T* __restrict__ coul, const T* __restrict__ x, T* __restrict__ y,
     const T* __restrict__ hii) {
   constexpr int KC = K * C;  // values per lane per ket row
@@ -173,7 +173,13 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
   if (col >= ncol) return;
   typename AccOf<T, kAcc>::type acc[C][K];
   const int64_t me = row * ncol + col;
-  if (kDiag) {
+  if (blk > 0) {  // a later ket block adds to the sums of the earlier ones
+    const VecK<T, KC> ym = ldk<T, KC>(y + me * K);
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[c][v].add(ym.v[c * K + v]);
+  } else if (kDiag) {
     const VecK<T, KC> xm = ldk<T, KC>(x + me * K);
     const VecK<T, C> h = ldk<T, C>(hii + me);
 #pragma unroll
@@ -189,7 +195,8 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
   };
 
   // same-spin doubles
-  const int64_t d0 = rt.off_d[row], d1 = rt.off_d[row + 1];
+  const int64_t rb = row * nblk + blk;
+  const int64_t d0 = rt.off_d[rb], d1 = rt.off_d[rb + 1];
   const T* __restrict__ xc = x + col * K;
   const int64_t stride = ncol * K;
   int64_t k = d0;
@@ -203,7 +210,7 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
   for (; k < d1; ++k) fma_all(val_d[k], ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_d[k]) * stride));
 
   // same-spin singles: value = stored part + sign * Coulomb table [pair, col]
-  const int64_t s0 = rt.off_s[row], s1 = rt.off_s[row + 1];
+  const int64_t s0 = rt.off_s[rb], s1 = rt.off_s[rb + 1];
   const T* __restrict__ cc = coul + col;
   auto pair_of = [&](int64_t q) {
     const uint32_t e = rt.exc_s[q];
@@ -687,16 +694,84 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   }
 }
 
-// the row kernel, one column per lane.  C = 16 / (K x word) consecutive columns per lane (16-byte
-// lanes, 64 C columns per wavefront) was measured slower at 4e8 determinants: the wider ket-row
-// segments double the L2 footprint of the rows in flight (FP64, one vector: 253 against 187 ms
-// per pass), so the kernel keeps C as a parameter and launches C = 1.
+// The rows in flight on an XCD reach nearly every ket row, so a row pass's L2 working set is
+// (ket rows) x (its 64-column segment of K words): 10 MB for 20,000 FP64 ket rows against an
+// XCD's 4 MB L2.  The pass is therefore cut into ket blocks whose segments fit in kRowL2Budget:
+// launch b sums the entries whose ket lies in block b and adds them to the earlier blocks'
+// sums (the diagonal and the first block write the output).
+constexpr size_t kRowL2Budget = size_t(3) << 20;
+constexpr int kMaxKetBlocks = 16;
+
+int ket_blocks_for(int64_t nket, int K, size_t word) {
+  const size_t ws = static_cast<size_t>(nket) * kRowTile * K * word;
+  return static_cast<int>(std::min<int64_t>(kMaxKetBlocks, std::max<int64_t>(1, cdiv(ws, kRowL2Budget))));
+}
+
+// first index of each (row, block) in a ket-sorted CSR list, blocks of bw kets
+__global__ __launch_bounds__(256) void k_ket_blocks(const int64_t* __restrict__ off, const uint32_t* __restrict__ ket,
+                                                    int64_t nrow, int64_t bw, int nblk, int64_t* __restrict__ out) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (q > nrow * nblk) return;
+  if (q == nrow * nblk) {
+    out[q] = off[nrow];
+    return;
+  }
+  const int64_t row = q / nblk, b = q - row * nblk;
+  int64_t lo = off[row], hi = off[row + 1];
+  const int64_t bound = b * bw;
+  while (lo < hi) {  // first entry with ket >= bound
+    const int64_t mid = (lo + hi) / 2;
+    if (static_cast<int64_t>(ket[mid]) < bound) lo = mid + 1;
+    else hi = mid;
+  }
+  out[q] = lo;
+}
+
+// the lists of one spin cut into nblk ket blocks (built once per block count)
+RowTabs ket_block_tabs(sbdx_ctx& c, SpinTab& t, int64_t nket, int nblk, hipStream_t st) {
+  RowTabs rt{t.off_s.get(), t.ket_s.get(), t.exc_s.get(), t.off_d.get(), t.ket_d.get()};
+  if (nblk == 1) return rt;
+  SpinTab::KetBlocks* kb = nullptr;
+  for (auto& k : t.kblk)
+    if (k.nblk == nblk) kb = &k;
+  if (!kb) {
+    if (c.no_alloc)
+      throw Error(SBDX_ERR_STATE, "the ket blocks of this precision's row passes are not built: run "
+                                  "sbdx_sigma_device once with this precision first");
+    for (auto& k : t.kblk)
+      if (k.nblk == 0) {
+        kb = &k;
+        break;
+      }
+    if (!kb) throw Error(SBDX_ERR_STATE, "no free slot for another ket-block count");
+    const int64_t n = t.n, bw = cdiv(nket, nblk);
+    kb->s.alloc(n * nblk + 1);
+    kb->d.alloc(n * nblk + 1);
+    const unsigned grid = static_cast<unsigned>(cdiv(n * nblk + 1, 256));
+    k_ket_blocks<<<grid, 256, 0, st>>>(t.off_s.get(), t.ket_s.get(), n, bw, nblk, kb->s.get());
+    k_ket_blocks<<<grid, 256, 0, st>>>(t.off_d.get(), t.ket_d.get(), n, bw, nblk, kb->d.get());
+    SBDX_CHECK_LAUNCH();
+    kb->nblk = nblk;
+  }
+  rt.off_s = kb->s.get();
+  rt.off_d = kb->d.get();
+  return rt;
+}
+
+// the row kernel, one column per lane, one launch per ket block.  C = 16 / (K x word)
+// consecutive columns per lane (16-byte lanes, 64 C columns per wavefront) was measured slower
+// at 4e8 determinants: the wider segments double the L2 working set (FP64, one vector: 253
+// against 187 ms per pass), so the kernel keeps C as a parameter and launches C = 1.
 template <typename T, bool kDiag, int K, int kAcc>
-void launch_rows(int64_t nrow, int64_t ncol, int L, const RowTabs& rt, const T* val_s, const T* val_d,
+void launch_rows(sbdx_ctx& c, SpinTab& t, int64_t nrow, int64_t ncol, const T* val_s, const T* val_d,
                  const T* coul, const T* x, T* y, const T* hii, hipStream_t st) {
-  k_sigma_rows<T, kDiag, K, kAcc, 1><<<static_cast<unsigned>(nrow * cdiv(ncol, kRowTile)), kRowTile, 0, st>>>(
-      nrow, ncol, L, rt, val_s, val_d, coul, x, y, hii);
-  SBDX_CHECK_LAUNCH();
+  const int nblk = ket_blocks_for(nrow, K, sizeof(T));  // kets of a row pass are its rows
+  const RowTabs rt = ket_block_tabs(c, t, nrow, nblk, st);
+  for (int b = 0; b < nblk; ++b) {
+    k_sigma_rows<T, kDiag, K, kAcc, 1><<<static_cast<unsigned>(nrow * cdiv(ncol, kRowTile)), kRowTile, 0, st>>>(
+        nrow, ncol, c.L, rt, nblk, b, val_s, val_d, coul, x, y, hii);
+    SBDX_CHECK_LAUNCH();
+  }
 }
 
 template <typename T, int K, int kAcc>
@@ -704,15 +779,13 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
              const T* vs_b, const T* vd_b, const T* V, const T* hii, const T* coul_a, const T* coul_b) {
   const int64_t na = c.na, nb = c.nb;
   const int64_t n = na * nb;
-  RowTabs ta{c.A.off_s.get(), c.A.ket_s.get(), c.A.exc_s.get(), c.A.off_d.get(), c.A.ket_d.get()};
-  RowTabs tb{c.B.off_s.get(), c.B.ket_s.get(), c.B.exc_s.get(), c.B.off_d.get(), c.B.ket_d.get()};
   Prof& pf = c.prof;
   auto mark = [&](int k) {
     if (pf.on) SBDX_HIP(hipEventRecord(pf.ev[k], st));
   };
   mark(0);
   // alpha rows: diagonal + alpha singles and doubles
-  launch_rows<T, true, K, kAcc>(na, nb, c.L, ta, vs_a, vd_a, coul_a, x, y, hii, st);
+  launch_rows<T, true, K, kAcc>(c, c.A, na, nb, vs_a, vd_a, coul_a, x, y, hii, st);
   SBDX_CHECK_LAUNCH();
   mark(1);
   // alpha single x beta single
@@ -730,7 +803,7 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   k_transpose<T, K><<<tblocks, 256, 0, st>>>(x, xT, na, nb);
   SBDX_CHECK_LAUNCH();
   mark(3);
-  launch_rows<T, false, K, kAcc>(nb, na, c.L, tb, vs_b, vd_b, coul_b, xT, yT, nullptr, st);
+  launch_rows<T, false, K, kAcc>(c, c.B, nb, na, vs_b, vd_b, coul_b, xT, yT, nullptr, st);
   SBDX_CHECK_LAUNCH();
   mark(4);
   k_add_transposed<T, K><<<tblocks, 256, 0, st>>>(yT, y, na, nb);

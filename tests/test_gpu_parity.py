@@ -203,3 +203,30 @@ def test_errors():
     e = Engine(4, np.array([3, 5], np.uint64), np.array([3], np.uint64))
     with pytest.raises(SbdxError):
         e.hdiag()  # integrals not set
+
+
+@pytest.fixture(scope="module")
+def tall():
+    """7,000 alpha x 48 beta strings of the 36-orbital family: the alpha-row pass's ket rows
+    (7,000 x 64 columns x word x block width) exceed the per-pass L2 budget, so it runs in ket
+    blocks (2 for one FP64 vector, 5 for a block of four); the beta pass runs in one."""
+    print("[case tall 7000x48]", flush=True)
+    inst = synth.make_custom(36, 8, 8, 72, 7000, 48, seed_ints=4)
+    return inst, O.System.from_instance(inst), Engine.from_instance(inst)
+
+
+@pytest.mark.parametrize("k,prec", [(1, 0), (1, 1), (2, 1), (4, 0)])
+def test_sigma_in_ket_blocks(tall, k, prec):
+    torch = pytest.importorskip("torch")
+    inst, so, eng = tall
+    X = np.random.default_rng(40 + k).standard_normal((so.ndet, k))
+    dt = torch.float64 if prec == 0 else torch.float32
+    dx = torch.tensor(X, dtype=dt, device="cuda").contiguous()
+    dy = torch.empty_like(dx)
+    eng.sigma_block_device(prec, k, dx.data_ptr(), dy.data_ptr())
+    Y = dy.cpu().numpy().astype(np.float64)
+    tol = 1e-12 if prec == 0 else 2e-6
+    for v in range(min(k, 2)):
+        ref = so.mult(X[:, v])
+        rel = np.max(np.abs(Y[:, v] - ref)) / np.max(np.abs(ref))
+        assert rel <= tol, (v, rel)
