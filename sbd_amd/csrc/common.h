@@ -87,7 +87,7 @@ struct SpinTab {
     int nblk = 0;
     DevBuf<int64_t> s, d;
   };
-  KetBlocks kblk[4];
+  KetBlocks kblk[6];
 };
 
 // Per-phase timing of sigma builds (HIP events on the launch stream), for measurement only.

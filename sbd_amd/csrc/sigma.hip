@@ -758,20 +758,28 @@ RowTabs ket_block_tabs(sbdx_ctx& c, SpinTab& t, int64_t nket, int nblk, hipStrea
   return rt;
 }
 
-// the row kernel, one column per lane, one launch per ket block.  C = 16 / (K x word)
-// consecutive columns per lane (16-byte lanes, 64 C columns per wavefront) was measured slower
-// at 4e8 determinants: the wider segments double the L2 working set (FP64, one vector: 253
-// against 187 ms per pass), so the kernel keeps C as a parameter and launches C = 1.
+// The row kernel, one launch per ket block.  Lane width: the vector memory path takes a lane
+// quad per clock for 8- and 16-byte loads and 16 lanes for 4-byte ones (a 64-lane load of
+// 8-byte values costs as many L1 accesses as one of 16-byte values), so K words of 8 bytes are
+// loaded two columns per lane (C = 2, 16-byte lanes, 128 columns per wavefront); 4- and 16-byte
+// K words keep one column per lane.  The ket blocks are sized for the C-column segments.
 template <typename T, bool kDiag, int K, int kAcc>
 void launch_rows(sbdx_ctx& c, SpinTab& t, int64_t nrow, int64_t ncol, const T* val_s, const T* val_d,
                  const T* coul, const T* x, T* y, const T* hii, hipStream_t st) {
-  const int nblk = ket_blocks_for(nrow, K, sizeof(T));  // kets of a row pass are its rows
-  const RowTabs rt = ket_block_tabs(c, t, nrow, nblk, st);
-  for (int b = 0; b < nblk; ++b) {
-    k_sigma_rows<T, kDiag, K, kAcc, 1><<<static_cast<unsigned>(nrow * cdiv(ncol, kRowTile)), kRowTile, 0, st>>>(
-        nrow, ncol, c.L, rt, nblk, b, val_s, val_d, coul, x, y, hii);
-    SBDX_CHECK_LAUNCH();
-  }
+  constexpr int CW = K * sizeof(T) == 8 ? 2 : 1;
+  auto run = [&](auto cc) {
+    constexpr int C = decltype(cc)::value;
+    const int nblk = ket_blocks_for(nrow, C * K, sizeof(T));  // kets of a row pass are its rows
+    const RowTabs rt = ket_block_tabs(c, t, nrow, nblk, st);
+    for (int b = 0; b < nblk; ++b) {
+      k_sigma_rows<T, kDiag, K, kAcc, C><<<static_cast<unsigned>(nrow * cdiv(ncol, kRowTile * C)), kRowTile, 0,
+                                           st>>>(nrow, ncol, c.L, rt, nblk, b, val_s, val_d, coul, x, y, hii);
+      SBDX_CHECK_LAUNCH();
+    }
+  };
+  // C columns per lane need whole 16-byte words per row
+  if (CW > 1 && ncol % CW == 0) run(std::integral_constant<int, CW>{});
+  else run(std::integral_constant<int, 1>{});
 }
 
 template <typename T, int K, int kAcc>
