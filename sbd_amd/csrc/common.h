@@ -119,6 +119,7 @@ struct Workspace {
   DevBuf<double> scal;           // device scalars
   DevBuf<unsigned char> xT, yT;  // transposed copies for the beta part (any precision)
   DevBuf<unsigned char> tmp_in, tmp_out;
+  DevBuf<unsigned char> sub_x, sub_y;  // one sub-block of a block built in sub-blocks
 };
 
 }  // namespace sbdx

@@ -504,13 +504,22 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
   stk<T, K>(y + k, a);
 }
 
-// variant of the mixed kernel for a block width K: the context's autotuned choice when there
-// is one (sbdx_autotune), else the autotuner's choice on the 4e8-determinant 36-orbital
-// workload, or for more than 45 orbitals (integral rows of more than 2048 values: the large
-// variants drop to one wave per SIMD) its choice on the 1e8-determinant 48-orbital workload
-int mix_variant(const sbdx_ctx& c, int prec, int K) {
+// A block of K = 4 or 8 vectors can also be built as sub-blocks of S = 2 (or 4) consecutive
+// vectors: the variant code S (kSplit2, kSplit4) of width K means "gather S of the K
+// interleaved values of every determinant into an S-wide block, build its sigma, scatter it
+// back", K / S times.  The fused K-wide kernels keep one K-value unit per lane and entry, so
+// their registers and L2 segments grow with K; at 4e8 determinants K = 2 sub-blocks cost less
+// per vector than the fused K = 4 and K = 8 builds.
+constexpr int kSplit2 = 2, kSplit4 = 4;
+
+// The mixed-kernel variant a fused K-wide build uses: the context's autotuned choice when there
+// is one (sbdx_autotune, sbdx_set_mix_variant), else the autotuner's choice on the
+// 4e8-determinant 36-orbital workload, or for more than 45 orbitals (integral rows of more than
+// 2048 values: the large variants drop to one wave per SIMD) its choice on the 1e8-determinant
+// 48-orbital workload.
+int fused_variant(const sbdx_ctx& c, int prec, int K) {
   const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
-  if (tuned) return tuned;
+  if (tuned >= 100) return tuned;  // a kernel variant, not a sub-block width
   if (c.L * c.L > 2048) {
     if (K == 1) return 25616;
     if (K == 2) return 25608;
@@ -519,6 +528,16 @@ int mix_variant(const sbdx_ctx& c, int prec, int K) {
   if (K == 2) return prec == 0 ? 102408 : 25616;
   if (K == 4) return prec == 0 ? 1102404 : 25608;
   return 25604;
+}
+
+// What a K-wide build runs: the tuned variant or sub-block width, else K = 2 sub-blocks for
+// K = 4 and 8 up to 45 orbitals (measured faster per vector at 4e8 determinants), else the
+// fused default.
+int mix_variant(const sbdx_ctx& c, int prec, int K) {
+  const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
+  if (tuned) return tuned;
+  if (K >= 4 && c.L * c.L <= 2048) return kSplit2;
+  return fused_variant(c, prec, K);
 }
 
 // Regroup the beta singles into mixed-pass tiles of at most cap entries (host side, once per
@@ -607,7 +626,7 @@ inline int mix_twosum_variant(const sbdx_ctx& c, int K) {
 template <typename T, int K, int kAcc>
 void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   const int var = kAcc == SBDX_ACC_TWOSUM ? mix_twosum_variant(c, K)
-                                          : mix_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
+                                          : fused_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
   const int stages = var / 1000000 + 1, threads = (var % 1000000) / 100;
   const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
   if (mt.ntile == 0) return;
@@ -828,6 +847,39 @@ void sigma_k(sbdx_ctx& c, const T* x, T* y, hipStream_t st, const T* vs_a, const
   }
 }
 
+// sub[d * S + v] = x[d * K + s0 + v] and back, v < S (one determinant per thread)
+template <typename T, int K, int S>
+__global__ __launch_bounds__(256) void k_sub_gather(const T* __restrict__ x, T* __restrict__ sub, int64_t n, int s0) {
+  const int64_t d = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (d < n) stk<T, S>(sub + d * S, ldk<T, S>(x + d * K + s0));
+}
+template <typename T, int K, int S>
+__global__ __launch_bounds__(256) void k_sub_scatter(const T* __restrict__ sub, T* __restrict__ y, int64_t n, int s0) {
+  const int64_t d = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (d < n) stk<T, S>(y + d * K + s0, ldk<T, S>(sub + d * S));
+}
+
+// a K-wide block built as K / S sub-blocks of width S (the kSplit variants)
+template <typename T, int K, int S, int kAcc, typename... Args>
+void sigma_split(sbdx_ctx& c, const T* x, T* y, hipStream_t st, Args... args) {
+  static_assert(S < K && K % S == 0, "sub-blocks divide the block");
+  if (c.no_alloc) throw Error(SBDX_ERR_STATE, "sub-block builds allocate their sub-block buffers");
+  const int64_t n = c.ndet();
+  c.ws.sub_x.ensure(n * S * sizeof(T));
+  c.ws.sub_y.ensure(n * S * sizeof(T));
+  T* sx = reinterpret_cast<T*>(c.ws.sub_x.get());
+  T* sy = reinterpret_cast<T*>(c.ws.sub_y.get());
+  const unsigned g = static_cast<unsigned>(cdiv(n, 256));
+  for (int s0 = 0; s0 < K; s0 += S) {
+    k_sub_gather<T, K, S><<<g, 256, 0, st>>>(x, sx, n, s0);
+    SBDX_CHECK_LAUNCH();
+    sigma_k<T, S, kAcc>(c, sx, sy, st, args...);
+    k_sub_scatter<T, K, S><<<g, 256, 0, st>>>(sy, y, n, s0);
+    SBDX_CHECK_LAUNCH();
+  }
+  if (c.prof.on) c.prof.calls -= K / S - 1;  // the phases of one block build, over its sub-blocks
+}
+
 template <typename T>
 void sigma_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
   const bool f = std::is_same<T, float>::value;
@@ -840,13 +892,21 @@ void sigma_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
   const T* hii = f ? reinterpret_cast<const T*>(c.hii32.get()) : reinterpret_cast<const T*>(c.hii.get());
   const T* ca = f ? reinterpret_cast<const T*>(c.coul_a32.get()) : reinterpret_cast<const T*>(c.coul_a.get());
   const T* cb = f ? reinterpret_cast<const T*>(c.coul_b32.get()) : reinterpret_cast<const T*>(c.coul_b.get());
+  const int split = k >= 4 ? mix_variant(c, f ? 1 : 0, k) : 0;
   auto go = [&](auto acc) {
     constexpr int A = decltype(acc)::value;
     switch (k) {
       case 1: sigma_k<T, 1, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
       case 2: sigma_k<T, 2, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
-      case 4: sigma_k<T, 4, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
-      case 8: sigma_k<T, 8, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb); break;
+      case 4:
+        if (split == kSplit2) sigma_split<T, 4, 2, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb);
+        else sigma_k<T, 4, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb);
+        break;
+      case 8:
+        if (split == kSplit2) sigma_split<T, 8, 2, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb);
+        else if (split == kSplit4) sigma_split<T, 8, 4, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb);
+        else sigma_k<T, 8, A>(c, x, y, st, vs_a, vd_a, vs_b, vd_b, V, hii, ca, cb);
+        break;
       default: throw Error(-1, "block width must be 1, 2, 4 or 8");
     }
   };
@@ -877,24 +937,25 @@ void reset_mix_tiles(sbdx_ctx& c) {
 // build (once) the mixed-pass tiles of the kernel variant a sigma build of precision prec and
 // block width k uses, so a later build does not allocate them
 void prepare_mix_tiles(sbdx_ctx& c, int prec, int k) {
-  const int var = mix_variant(c, prec, k);
+  int var = mix_variant(c, prec, k);
+  if (var < k) var = fused_variant(c, prec, var);  // sub-blocks of width var, fused
   const int threads = (var % 1000000) / 100;
   (void)mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
 }
 
 // variants (1000000 (stages - 1) + 100 threads + entries per thread) of the mixed kernel for
-// block width K
+// block width K, and for K = 4 and 8 the sub-block widths (kSplit2, kSplit4)
 std::vector<int> mix_variant_candidates(int K) {
   if (K == 1) return {25632, 25616, 51216, 102408, 1102408, 3102408, 1051216, 1025616};
   if (K == 2) return {102404, 25608, 102408, 25616, 51216, 1102404, 3102404, 1051208};
-  if (K == 4) return {25608, 25604, 102404, 1102404};
-  return {25604, 51204};
+  if (K == 4) return {25608, 25604, 102404, 1102404, kSplit2};
+  return {25604, 51204, kSplit2, kSplit4};
 }
 
 // whether a variant's LDS (double-buffered V rows of its stages, reused by the fold for one
 // vector's entry sums) fits a CU for this problem
 bool mix_variant_fits(const sbdx_ctx& c, int prec, int K, int var) {
-  (void)K;
+  if (var < K) return mix_variant_fits(c, prec, var, fused_variant(c, prec, var));  // fused sub-blocks
   const int stages = var / 1000000 + 1, es = prec == 0 ? 8 : 4;
   const int threads = (var % 1000000) / 100, ept = var % 100;
   return mix_lds_bytes(c.L, c.A.max_s, stages, es, threads * ept, 8) <= kMixMaxLds;

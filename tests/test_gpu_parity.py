@@ -94,15 +94,16 @@ def test_sigma_block_interleaved(pair, k, prec):
 
 VARIANTS = {1: [25632, 25616, 51216, 102408, 1102408, 3102408, 1051216, 1025616],
             2: [102404, 25608, 102408, 25616, 51216, 1102404, 3102404, 1051208],
-            4: [25608, 25604, 102404, 1102404], 8: [25604, 51204]}
+            4: [25608, 25604, 102404, 1102404, 2], 8: [25604, 51204, 2, 4]}
 
 
 @pytest.mark.parametrize("k", [1, 2, 4, 8])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_every_mixed_kernel_variant(pair, k, prec):
     """Each variant the autotuner may pick (threads per workgroup x entries per thread of the
-    alpha-single x beta-single kernel) gives the oracle's sigma; then the autotuner's own
-    choice does, and variant 0 restores the default."""
+    alpha-single x beta-single kernel; for k = 4 and 8 also the sub-block widths 2 and 4, the
+    block built as k / width narrower blocks) gives the oracle's sigma; then the autotuner's
+    own choice does, and variant 0 restores the default."""
     torch = pytest.importorskip("torch")
     name, inst, so, eng = pair
     if name != "c2fam_300x280" and not (name == "l10_3a3b_samp" and k <= 2):
