@@ -9,7 +9,7 @@
 //
 // Reference interfaces (include/sbd/ in r-ccs-cms/sbd):
 //   oneInt, twoInt                 chemistry/basic/integrals.h:10-67
-//   FCIDump, LoadFCIDump           framework/fcidump.h:18-139
+//   FCIDump, LoadFCIDump, printFCIDump  framework/fcidump.h:18-152
 //   SetupIntegrals                 chemistry/basic/makeintegrals.h:10-55
 //   from_string, makestring,       framework/bit_manipulation.h:135-172, :324-330
 //   sort_bitarray, LoadAlphaDets   chemistry/basic/makedeterminants.h:160-185 (.txt and .bin)
@@ -154,6 +154,18 @@ inline FCIDump LoadFCIDump(const std::string& filename) {
     d.header[trim(item.substr(0, eq))] = trim(item.substr(eq + 1));
   }
   return d;
+}
+
+// The header ("key = value" per line, in key order) and the integral records ("value i j k l"
+// per line, values at the stream's default 6 significant digits) on standard output
+// (framework/fcidump.h:141-152).
+inline void printFCIDump(const FCIDump& fcidump) {
+  std::cout << "Header:" << std::endl;
+  for (const auto& kv : fcidump.header) std::cout << kv.first << " = " << kv.second << std::endl;
+  std::cout << "\nIntegrals:" << std::endl;
+  for (const auto& r : fcidump.integrals)
+    std::cout << std::get<0>(r) << " " << std::get<1>(r) << " " << std::get<2>(r) << " " << std::get<3>(r) << " "
+              << std::get<4>(r) << std::endl;
 }
 
 template <typename ElemT>

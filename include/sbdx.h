@@ -258,8 +258,10 @@ void sbdx_batch_destroy(sbdx_batch *batch);
 /* Autotuner, no counterpart in the reference: times every variant of the sigma build's
  * alpha-single x beta-single kernel (coded 1000000 (stages - 1) + 100 * threads per workgroup +
  * entries per thread: threads x entries set its tile shape, stages the alpha singles taken per
- * barrier) that fits this problem, for block width k and precision prec on this context's
- * problem, `reps` builds each, and keeps the fastest for every later build of that (prec, k).
+ * barrier) that fits this problem, and for k = 4 and 8 the sub-block widths (codes 2 and 4: the
+ * block built as k / code blocks of that width, each gathered out of the interleaved block and
+ * scattered back), for block width k and precision prec on this context's problem, `reps`
+ * builds each, and keeps the fastest for every later build of that (prec, k).
  * candidates/ms (optional, cap entries): the variants and their mean ms per build; *chosen:
  * the variant kept; *ncand: variants tried. */
 int sbdx_autotune(sbdx_ctx *ctx, int prec, int k, int reps, int *chosen, int *candidates,

@@ -6,6 +6,8 @@
 //       LoadFCIDump + SetupIntegrals: <out>.header.txt ("key\tvalue" lines), <out>.ints.f64,
 //       <out>.ints.i32 (i j k l per record), <out>.I0.f64, <out>.I1.f64, <out>.I2.f64; on a
 //       std::exception prints {"error": ...} and exits 4
+//   shim_post print <FCIDUMP file>
+//       LoadFCIDump + printFCIDump: the printed header and records on standard output
 //   shim_post carry <alpha u64> <beta u64> <norb> <W f64> <out prefix> [--threshold t]... [--kept k]...
 //       CarryOverAlphaDet in both forms: <out>.carry.i64 (per case: count, then the indices of
 //       the selected strings in the alpha table), <out>.discarded.f64 (kept form)
@@ -124,8 +126,11 @@ int main(int argc, char** argv) {
   const std::string op = argc > 1 ? argv[1] : "";
   try {
     if (op == "fcidump" && argc == 4) rc = run_fcidump(argv[2], argv[3]);
-    else if (op == "carry" && argc >= 7) rc = run_carry(argc, argv);
-    else std::fprintf(stderr, "usage: shim_post fcidump <file> <out> | carry <alpha> <beta> <norb> <W> <out> ...\n");
+    else if (op == "print" && argc == 3) {
+      sbd::printFCIDump(sbd::LoadFCIDump(argv[2]));
+      rc = 0;
+    } else if (op == "carry" && argc >= 7) rc = run_carry(argc, argv);
+    else std::fprintf(stderr, "usage: shim_post fcidump <file> <out> | print <file> | carry <alpha> <beta> <norb> <W> <out> ...\n");
   } catch (const std::exception& e) {
     std::printf("{\"error\":\"%s\"}\n", e.what());
     rc = 4;

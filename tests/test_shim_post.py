@@ -59,6 +59,23 @@ def test_fcidump_matches_reference(shim_post, golden, tmp_path, name):
         assert np.array_equal(np.fromfile(f"{out}.{t}.f64", np.float64), z[f"fcidump_{name}_{t}"]), t
 
 
+@pytest.mark.parametrize("name", ["std", "slash", "joined"])
+def test_print_fcidump(shim_post, golden, name):
+    """printFCIDump (framework/fcidump.h:141-152): "Header:", one "key = value" line per header
+    entry in key order, a blank line, "Integrals:", one "value i j k l" line per record with the
+    value at the stream's default 6 significant digits (C's %g)."""
+    meta, z = golden
+    r = subprocess.run([shim_post, "print", os.path.join(GOLDEN, "fcidump", f"{name}.FCIDUMP")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    header = [ln.split("\t", 1) for ln in str(z[f"fcidump_{name}_header"]).splitlines()]
+    vals = z[f"fcidump_{name}_vals"]
+    idx = z[f"fcidump_{name}_idx"].reshape(-1, 4)
+    want = ["Header:"] + [f"{k} = {v}" for k, v in header] + ["", "Integrals:"]
+    want += ["%g %d %d %d %d" % (v, *q) for v, q in zip(vals, idx)]
+    assert r.stdout.splitlines() == want
+
+
 def test_fcidump_layout_by_hand(golden):
     """What the fixtures say, restated: ORBSYM keeps its first value, a '/' terminator leaves
     every line in the header (no integrals), a key whose line has no trailing comma swallows
