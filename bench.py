@@ -64,6 +64,8 @@ SOLVERS = {
                                   mlp="sbd_amd/weights/c2_family.npy"),
     # eight lowest roots in one block of eight
     "mixed_roots8": dict(method=1, precision=1, nroots=8, num_block=4),
+    # the same with 5 blocks (fits 288 GB since blocks of 8 are built as two-vector sub-blocks)
+    "mixed_roots8_nb5": dict(method=1, precision=1, nroots=8, num_block=5),
 }
 
 

@@ -324,7 +324,8 @@ class Engine:
     def autotune(self, prec, k, reps=2):
         """Time the mixed-pass variants for (prec, k) and keep the fastest; returns
         (chosen variant, {variant: mean ms}), variant = 1000000 (stages - 1) + 100 * threads
-        per workgroup + entries per thread."""
+        per workgroup + entries per thread, or for k = 4 and 8 a sub-block width (2 or 4: the
+        block built as k / width narrower blocks)."""
         cand = np.zeros(16, np.int32)
         ms = np.zeros(16)
         chosen, nc = C.c_int(), C.c_int()
