@@ -143,7 +143,7 @@ typedef struct {
 
 typedef struct {
   double energy;          /* <W|H|W> of the returned vector (reference: sbdiag.h:239-262) */
-  double residual;        /* last ||r|| */
+  double residual;        /* last ||r||, the largest over the requested roots */
   int steps;              /* sigma builds inside the iteration */
   int converged;
   double seconds;         /* wall time of the iteration */

@@ -831,14 +831,19 @@ struct BlockSolver {
           if (resid) resid[nstep] = nr[0];
         }
         ++nstep;
-        if (o.verbose) {
-          std::printf(" Davidson iteration %d.%d (tol=%.16g)%s:", it, nblocks - 1, nr[0], exact ? "" : " (mixed)");
-          for (int v = 0; v < std::min(K, 4); ++v) std::printf(" %.16g", E[v]);
-          std::printf("\n");
-          std::fflush(stdout);
-        }
         double worst = 0.0;
         for (int v = 0; v < nroots; ++v) worst = std::max(worst, nr[v]);
+        if (o.verbose) {
+          std::printf(" Davidson iteration %d.%d (tol=%.16g)%s:", it, nblocks - 1, nr[0], exact ? "" : " (mixed)");
+          for (int v = 0; v < std::min(K, std::max(4, nroots)); ++v) std::printf(" %.16g", E[v]);
+          std::printf("\n");
+          if (exact && nroots > 1) {  // every requested root's residual on the exact steps
+            std::printf("   residuals:");
+            for (int v = 0; v < nroots; ++v) std::printf(" %.3e", nr[v]);
+            std::printf("\n");
+          }
+          std::fflush(stdout);
+        }
         const bool near_ref = o.energy_ref != 0.0 && std::fabs(E[0] - o.energy_ref) < o.energy_tol;
         if (exact) {
           // stall test of the learned preconditioner on the exact residual of every restart:
@@ -901,7 +906,9 @@ struct BlockSolver {
     SBDX_HIP(hipStreamSynchronize(st));
     res.seconds = now() - t0;
     res.steps = nstep;
-    res.residual = nr[0];
+    double worst = 0.0;
+    for (int v = 0; v < nroots; ++v) worst = std::max(worst, nr[v]);
+    res.residual = worst;
     res.converged = conv ? 1 : 0;
     res.energy = E[0];
     res.precond_fallback = fallback_step;
