@@ -249,6 +249,34 @@ def rdm_after_solve(eng, inst, W, name, energy):
     return rec
 
 
+def rdm_random(eng, inst, g):
+    """1- and 2-RDM of a random normalized vector (device resident), timed; the energy identity
+    against <W|H|W> from one FP64 sigma of the same vector holds for any normalized W."""
+    import torch
+    N = eng.ndet
+    W = torch.randn(N, dtype=torch.float64, device="cuda", generator=g)
+    W /= torch.linalg.vector_norm(W)
+    y = torch.empty_like(W)
+    eng.sigma_device(0, 1, W.data_ptr(), y.data_ptr())
+    torch.cuda.synchronize()  # the build runs on the context's stream
+    e_hw = float(torch.dot(W, y))
+    del y
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    one, two = eng.rdm_device(W.data_ptr())
+    dt = time.perf_counter() - t
+    L = inst.norb
+    tr = [float(np.trace(one[s].reshape(L, L))) for s in (0, 1)]
+    e = rdm_energy(inst, one, two)
+    rec = dict(vector="random, normalized", seconds=round(dt, 3), trace_alpha=tr[0], trace_beta=tr[1],
+               electrons=[inst.n_alpha_el, inst.n_beta_el], energy_from_rdm=e, energy_hw=e_hw,
+               energy_identity_error=abs(e - e_hw))
+    log(f"[bench] rdm (random vector): {rec}")
+    del W
+    torch.cuda.empty_cache()
+    return rec
+
+
 def precision_report(eng, max_time):
     """The workload solved to residual 1e-8 twice -- the reference algorithm in FP64 and the
     mixed-precision block solver (FP32 corrections, FP64 anchor refinement) -- and the errors of
@@ -333,6 +361,9 @@ def main():
                     help="skip the default time-to-1e-6 Ha solves (c2, FP64 runs only)")
     ap.add_argument("--no-rdm", action="store_true",
                     help="skip the 1- and 2-RDM of the last solve's vector")
+    ap.add_argument("--rdm-random", action="store_true",
+                    help="time the 1- and 2-RDM of a random normalized vector (energy identity "
+                         "against <W|H|W>)")
     ap.add_argument("--plain-steps", type=int, default=20,
                     help="steps of the bounded plain FP64 run that gives its seconds per step")
     ap.add_argument("--converge", action="store_true",
@@ -586,6 +617,9 @@ def main():
         if t.get("plain") and t["plain"] > 0:
             out["solve"]["speedup_vs_plain"] = {n: round(t["plain"] / v, 3) for n, v in t.items()
                                                 if v and v > 0 and n != "plain"}
+
+    if args.rdm_random:
+        out["rdm_random"] = rdm_random(eng, inst, g)
 
     if args.precision_report:
         out["precision_report"] = precision_report(eng, args.solve_time)
