@@ -85,37 +85,75 @@ __global__ __launch_bounds__(kB) void k_rowstats(const double* __restrict__ W, i
   }
 }
 
-// One workgroup per row, one wavefront per listed ket (the row's kets dealt round-robin over
-// the four wavefronts): rho_k = sum_col W[row, col] W[ket_k, col] and, with kBins,
+// One workgroup per row, one wavefront per G listed kets (the row's kets dealt round-robin over
+// the four wavefronts, G at a time): rho_k = sum_col W[row, col] W[ket_k, col] and, with kBins,
 // rho_kq = the same weighted by n_q(col) for q < L.  out[k * (1 + L)] = rho_k,
-// out[k * (1 + L) + 1 + q] = rho_kq (stride 1 without bins).  No barriers, no atomics.
-template <int LQ, bool kBins>
+// out[k * (1 + L) + 1 + q] = rho_kq (stride 1 without bins).  Each lane takes C consecutive
+// columns (16-byte loads for C = 2) and the G kets share every load of the bra row.  No
+// barriers, no atomics.
+// C consecutive doubles (C = 2: one 16-byte load; p 16-byte aligned)
+template <int C>
+struct Cols {
+  double v[C];
+};
+template <int C>
+__device__ inline Cols<C> ld_cols(const double* p) {
+  Cols<C> r;
+  if constexpr (C == 2) {
+    const double2 t = *reinterpret_cast<const double2*>(p);
+    r.v[0] = t.x;
+    r.v[1] = t.y;
+  } else {
+    r.v[0] = *p;
+  }
+  return r;
+}
+
+template <int LQ, bool kBins, int G, int C>
 __global__ __launch_bounds__(kB) void k_pairdots(const double* __restrict__ W, int64_t ncol,
                                                  const int64_t* __restrict__ off,
                                                  const uint32_t* __restrict__ ket,
                                                  const uint64_t* __restrict__ col_str, int L,
                                                  double* __restrict__ out) {
+  static_assert(!kBins || G == 1, "the weighted sums keep one ket's bins per lane");
   const int64_t row = blockIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int stride = kBins ? 1 + L : 1;
   const double* __restrict__ wr = W + row * ncol;
-  for (int64_t k = off[row] + wv; k < off[row + 1]; k += kB / 64) {
-    const double* __restrict__ wk = W + static_cast<int64_t>(ket[k]) * ncol;
-    double s = 0.0;
+  const int64_t kend = off[row + 1];
+  for (int64_t k0 = off[row] + wv * G; k0 < kend; k0 += (kB / 64) * G) {
+    const double* __restrict__ wk[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g)  // past the row's kets: reread the first one, drop its sum
+      wk[g] = W + static_cast<int64_t>(ket[k0 + g < kend ? k0 + g : k0]) * ncol;
+    double s[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) s[g] = 0.0;
     Bins<kBins ? LQ : 1> bins;
     bins.zero();
-    for (int64_t col = lane; col < ncol; col += 64) {
-      const double p = wr[col] * wk[col];
-      s += p;
-      if (kBins) bins.add(col_str[col], p);
+    for (int64_t col = static_cast<int64_t>(lane) * C; col < ncol; col += 64 * C) {
+      const Cols<C> a = ld_cols<C>(wr + col);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const Cols<C> b = ld_cols<C>(wk[g] + col);
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+          const double p = a.v[c] * b.v[c];
+          s[g] += p;
+          if constexpr (kBins) bins.add(col_str[col + c], p);
+        }
+      }
     }
-    s = wave_sum(s);
-    if (lane == 0) out[k * stride] = s;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const double v = wave_sum(s[g]);
+      if (lane == 0 && k0 + g < kend) out[(k0 + g) * stride] = v;
+    }
     if (kBins) {
 #pragma unroll
       for (int q = 0; q < (kBins ? LQ : 1); ++q) {
         const double v = wave_sum(bins.b[q]);
-        if (lane == 0 && q < L) out[k * stride + 1 + q] = v;
+        if (lane == 0 && q < L) out[k0 * stride + 1 + q] = v;
       }
     }
   }
@@ -464,10 +502,16 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
     k_rowstats<LQ><<<ga, kB, 0, st>>>(dW, nb, c.B.str.get(), L, rowsq.get(), ta.get());
     k_rowstats<1><<<gb, kB, 0, st>>>(wT.get(), na, c.A.str.get(), L, colsq.get(), nullptr);
     SBDX_CHECK_LAUNCH();
-    k_pairdots<LQ, true><<<ga, kB, 0, st>>>(dW, nb, c.A.off_s.get(), c.A.ket_s.get(), c.B.str.get(), L, rs_a.get());
-    k_pairdots<1, false><<<ga, kB, 0, st>>>(dW, nb, c.A.off_d.get(), c.A.ket_d.get(), c.B.str.get(), L, rd_a.get());
-    k_pairdots<LQ, true><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_s.get(), c.B.ket_s.get(), c.A.str.get(), L, rs_b.get());
-    k_pairdots<1, false><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_d.get(), c.B.ket_d.get(), c.A.str.get(), L, rd_b.get());
+    k_pairdots<LQ, true, 1, 1><<<ga, kB, 0, st>>>(dW, nb, c.A.off_s.get(), c.A.ket_s.get(), c.B.str.get(), L, rs_a.get());
+    k_pairdots<LQ, true, 1, 1><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_s.get(), c.B.ket_s.get(), c.A.str.get(), L, rs_b.get());
+    // the same-spin doubles: four kets per wavefront, two columns per lane where the rows hold
+    // whole 16-byte words
+    auto doubles = [&](unsigned g, const double* w, int64_t ncol, const SpinTab& t, double* out) {
+      if (ncol % 2 == 0) k_pairdots<1, false, 4, 2><<<g, kB, 0, st>>>(w, ncol, t.off_d.get(), t.ket_d.get(), nullptr, L, out);
+      else k_pairdots<1, false, 4, 1><<<g, kB, 0, st>>>(w, ncol, t.off_d.get(), t.ket_d.get(), nullptr, L, out);
+    };
+    doubles(ga, dW, nb, c.A, rd_a.get());
+    doubles(gb, wT.get(), na, c.B, rd_b.get());
     SBDX_CHECK_LAUNCH();
   };
   if (L <= 32) stats(std::integral_constant<int, 32>{});
