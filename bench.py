@@ -56,6 +56,8 @@ SOLVERS = {
     "mixed_block2_nb16": dict(method=1, precision=1, block=2, num_block=16),
     "mixed_block2_nb20": dict(method=1, precision=1, block=2, num_block=20),
     "mixed_block2_nb24": dict(method=1, precision=1, block=2, num_block=24),
+    "mixed_block2_nb28": dict(method=1, precision=1, block=2, num_block=28),
+    "mixed_block2_nb32": dict(method=1, precision=1, block=2, num_block=32),
     "block2_fp64_nb16": dict(method=1, precision=0, block=2, num_block=16),
     # the learned preconditioner trained on 36-orbital family instances (sbd_amd/mlp_train.py)
     "mixed_block2_nb16_mlp": dict(method=1, precision=1, block=2, num_block=16, preconditioner=1,
