@@ -188,29 +188,39 @@ void SetupIntegrals(const FCIDump& fcidump, int& L, int& N, ElemT& I0, oneInt<El
                     (in(i) && in(j) && in(k) && in(l));
     if (!ok) throw std::out_of_range("SetupIntegrals: integral record with an index outside 1..NORB");
   }
+  // Storage (chemistry/basic/integrals.h:10-67): I1 over spin orbitals 2p + s, row-major, with
+  // the same h_pq in both spin blocks and both orders; I2 once per spatial quadruple, at the
+  // packed slot of the pair of packed pairs (8-fold symmetry); the all-zero record is the core
+  // energy.  Records are applied in file order, so a repeated record keeps its last value.
   I1.norbs = 2 * L;
-  I1.store.assign(4 * L * L, ElemT(0.0));
-  int B = L * (L + 1) / 2;
+  I1.store.assign(static_cast<size_t>(4) * L * L, ElemT(0.0));
+  const size_t npair = static_cast<size_t>(L) * (L + 1) / 2;
   I2.norbs = L;
-  I2.store.assign(static_cast<size_t>(B) * (B + 1) / 2, ElemT(0.0));
-  for (const auto& [value, i, j, k, l] : fcidump.integrals) {
-    if (i == 0 && j == 0 && k == 0 && l == 0) {
-      I0 = ElemT(value);
-    } else if (k == 0 && l == 0) {
-      I1(2 * (i - 1), 2 * (j - 1)) = ElemT(value);
-      I1(2 * (i - 1) + 1, 2 * (j - 1) + 1) = ElemT(value);
-      I1(2 * (j - 1), 2 * (i - 1)) = ElemT(value);
-      I1(2 * (j - 1) + 1, 2 * (i - 1) + 1) = ElemT(value);
+  I2.store.assign(npair * (npair + 1) / 2, ElemT(0.0));
+  auto packed = [](size_t a, size_t b) { return a >= b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; };
+  auto eri_slot = [&](int p, int q, int r, int t) { return packed(packed(p, q), packed(r, t)); };
+  for (const auto& rec : fcidump.integrals) {
+    const ElemT v = ElemT(std::get<0>(rec));
+    const int p = std::get<1>(rec) - 1, q = std::get<2>(rec) - 1;
+    const int r = std::get<3>(rec) - 1, t = std::get<4>(rec) - 1;
+    if (p < 0 && q < 0 && r < 0 && t < 0) {
+      I0 = v;
+    } else if (r < 0 && t < 0) {
+      for (int spin = 0; spin < 2; ++spin) {
+        I1.store[static_cast<size_t>(2 * p + spin) * I1.norbs + 2 * q + spin] = v;
+        I1.store[static_cast<size_t>(2 * q + spin) * I1.norbs + 2 * p + spin] = v;
+      }
     } else {
-      I2(2 * (i - 1), 2 * (j - 1), 2 * (k - 1), 2 * (l - 1)) = ElemT(value);
+      I2.store[eri_slot(p, q, r, t)] = v;
     }
   }
-  I2.DirectMat.resize(L * L);
-  I2.ExchangeMat.resize(L * L);
-  for (int i = 0; i < L; i++)
-    for (int j = 0; j < L; j++) {
-      I2.Direct(i, j) = I2(2 * i, 2 * i, 2 * j, 2 * j);
-      I2.Exchange(i, j) = I2(2 * i, 2 * j, 2 * j, 2 * i);
+  // Coulomb J_pq = (pp|qq) and exchange K_pq = (pq|qp), column-major L x L
+  I2.DirectMat.assign(static_cast<size_t>(L) * L, ElemT(0.0));
+  I2.ExchangeMat.assign(static_cast<size_t>(L) * L, ElemT(0.0));
+  for (int q = 0; q < L; ++q)
+    for (int pp = 0; pp < L; ++pp) {
+      I2.DirectMat[pp + static_cast<size_t>(L) * q] = I2.store[eri_slot(pp, pp, q, q)];
+      I2.ExchangeMat[pp + static_cast<size_t>(L) * q] = I2.store[eri_slot(pp, q, q, pp)];
     }
 }
 
