@@ -56,6 +56,9 @@ SOLVERS = {
     "mixed_block2_nb16": dict(method=1, precision=1, block=2, num_block=16),
     "mixed_block2_nb20": dict(method=1, precision=1, block=2, num_block=20),
     "mixed_block2_nb24": dict(method=1, precision=1, block=2, num_block=24),
+    # FP32 sigma with compensated two-sum FP32 partials instead of FP64 accumulation of the
+    # exact FP32 products
+    "mixed_block2_nb24_twosum": dict(method=1, precision=1, block=2, num_block=24, fp32_acc=1),
     "mixed_block2_nb28": dict(method=1, precision=1, block=2, num_block=28),
     "mixed_block2_nb32": dict(method=1, precision=1, block=2, num_block=32),
     "block2_fp64_nb16": dict(method=1, precision=0, block=2, num_block=16),
@@ -573,6 +576,9 @@ def main():
                         verbose=int(args.verbose))
             weights = opts.pop("mlp", None)
             eng.mlp_set(np.load(os.path.join(ROOT, weights)) if weights else None)
+            acc = opts.pop("fp32_acc", None)
+            if acc is not None:
+                eng.set_fp32_accumulation(acc)
             if e_ref is not None:
                 opts.update(energy_ref=e_ref, energy_tol=1e-6, stop_at_tol=1)
             k = max(1, opts.get("nroots", 1))
@@ -586,6 +592,8 @@ def main():
                 residual=r["residual"], energy=r["energy"],
                 min_ritz=float(tr.min()) if tr.size else None)
             log(f"[bench] solve {name}: {out['solve'][name]}")
+            if acc is not None:
+                eng.set_fp32_accumulation(1 if args.fp32_acc == "twosum" else 0)
             if want_rdm and name == names[-1]:
                 out["rdm"] = rdm_after_solve(eng, inst, W, name, r["energy"])
             del W
