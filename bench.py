@@ -75,8 +75,9 @@ SOLVERS = {
 
 
 # the default run's solves: mixed precision block Davidson with and without the learned
-# preconditioner (the plain FP64 solver is timed by a bounded run; see plain_full_run)
-DEFAULT_SOLVES = ("mixed_block2_nb16_mlp", "mixed_block2_nb24")
+# preconditioner, and with the FP32 sigma's two-sum accumulation (the plain FP64 solver is timed
+# by a bounded run; see plain_full_run)
+DEFAULT_SOLVES = ("mixed_block2_nb16_mlp", "mixed_block2_nb24_twosum", "mixed_block2_nb24")
 DEFAULT_SOLVE_TIME = 180.0
 
 
