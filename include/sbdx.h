@@ -48,8 +48,9 @@ enum { SBDX_FP64 = 0, SBDX_FP32 = 1 };
 /* Accumulation of FP32 sigma builds (FP32 integrals and vectors either way).
  * SBDX_ACC_FP64: each product of two FP32 numbers is exact in FP64 and is added unrounded into
  *   an FP64 accumulator, rounded to FP32 once per output (the default; fewer instructions).
- * SBDX_ACC_TWOSUM: FP32 accumulation with compensated partials, an error-free product (FMA
- *   residual) and a TwoSum per term (Ogita-Rump-Oishi Dot2). */
+ * SBDX_ACC_TWOSUM: FP32 accumulation with compensated partials: runs of up to 16 terms are
+ *   summed by FP32 FMA, and each run is added to the running sum by TwoSum, its rounding
+ *   errors gathered in a compensation term. */
 enum { SBDX_ACC_FP64 = 0, SBDX_ACC_TWOSUM = 1 };
 
 typedef struct sbdx_ctx sbdx_ctx;

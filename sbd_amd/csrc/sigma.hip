@@ -21,8 +21,8 @@
 // Arithmetic: FP64 (FMA), or FP32 data whose products (exact in FP64) are accumulated in FP64
 // and rounded to FP32 once per pass (alpha rows, mixed, beta rows) when the pass adds its sum
 // to the output element, or (SBDX_ACC_TWOSUM) FP32 data accumulated in FP32 with compensated
-// partials (error-free product + TwoSum per term).  Every output element is summed in a fixed
-// order, so a build is bitwise reproducible.
+// partials (runs of up to 16 FMA terms, each run added to the sum by TwoSum).  Every output
+// element is summed in a fixed order, so a build is bitwise reproducible.
 #include "common.h"
 
 #include <vector>
@@ -45,6 +45,7 @@ struct AccD {
   double s = 0.0;
   __device__ inline void add(double p) { s += p; }
   __device__ inline void fma(double a, double b) { s = __fma_rn(a, b, s); }
+  __device__ inline void flush() {}
   __device__ inline double get() const { return s; }
   __device__ inline Wide wide() const { return s; }
   __device__ inline double apply(double y) const { return y + s; }
@@ -63,30 +64,35 @@ struct AccX {
     s = __fma_rn(static_cast<double>(a), static_cast<double>(b), s);
   }
   __device__ inline void add(double p) { s += p; }
+  __device__ inline void flush() {}
   __device__ inline float get() const { return static_cast<float>(s); }
   __device__ inline Wide wide() const { return s; }
   __device__ inline float apply(float y) const { return static_cast<float>(static_cast<double>(y) + s); }
 };
 
-// compensated FP32 (SBDX_ACC_TWOSUM): p = a*b exactly as p + e (FMA residual), s + p exactly
-// as t + err (TwoSum), errors gathered in c (Ogita-Rump-Oishi Dot2)
+// compensated FP32 (SBDX_ACC_TWOSUM): the terms of a short run (one batch of gathers, at most
+// 16 terms) are summed with FMA into the FP32 partial p, and flush() adds p to the running sum
+// s exactly, as s' + err (TwoSum), the rounding errors gathered in c.  Partials from elsewhere
+// (the previous ket block's sum, the mixed pass's per-entry sums) enter the same way.  One FMA
+// per term and one TwoSum per run.
 struct AccF {
   using Wide = float;
-  float s = 0.0f, c = 0.0f;
-  __device__ inline void add(float p) {
-    const float t = __fadd_rn(s, p);
-    const float bp = __fsub_rn(t, s);
-    c += __fadd_rn(__fsub_rn(s, __fsub_rn(t, bp)), __fsub_rn(p, bp));
+  float s = 0.0f, c = 0.0f, p = 0.0f;
+  __device__ inline void add(float q) {
+    const float t = __fadd_rn(s, q);
+    const float bq = __fsub_rn(t, s);
+    c += __fadd_rn(__fsub_rn(s, __fsub_rn(t, bq)), __fsub_rn(q, bq));
     s = t;
   }
-  __device__ inline void fma(float a, float b) {
-    const float p = __fmul_rn(a, b);
-    c += __fmaf_rn(a, b, -p);
+  __device__ inline void fma(float a, float b) { p = __fmaf_rn(a, b, p); }
+  __device__ inline void flush() {
     add(p);
+    p = 0.0f;
   }
-  __device__ inline float get() const { return s + c; }
-  __device__ inline Wide wide() const { return s + c; }
+  __device__ inline float get() const { return s + (c + p); }
+  __device__ inline Wide wide() const { return s + (c + p); }
   __device__ inline float apply(float y) {
+    flush();
     add(y);
     return get();
   }
@@ -193,6 +199,13 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
 #pragma unroll
       for (int v = 0; v < K; ++v) acc[c][v].fma(w, xv.v[c * K + v]);
   };
+  auto flush_all = [&] {  // the compensated FP32 accumulation closes a run of terms
+#pragma unroll
+    for (int c = 0; c < C; ++c)
+#pragma unroll
+      for (int v = 0; v < K; ++v) acc[c][v].flush();
+  };
+  flush_all();
 
   // same-spin doubles
   const int64_t rb = row * nblk + blk;
@@ -206,8 +219,10 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
     for (int u = 0; u < U; ++u) xv[u] = ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_d[k + u]) * stride);
 #pragma unroll
     for (int u = 0; u < U; ++u) fma_all(val_d[k + u], xv[u]);
+    flush_all();
   }
   for (; k < d1; ++k) fma_all(val_d[k], ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_d[k]) * stride));
+  flush_all();
 
   // same-spin singles: value = stored part + sign * Coulomb table [pair, col]
   const int64_t s0 = rt.off_s[rb], s1 = rt.off_s[rb + 1];
@@ -238,9 +253,11 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) fma_single(k + u, dv[u], xv[u]);
+    flush_all();
   }
   for (; k < s1; ++k)
     fma_single(k, ldk<T, C>(cc + pair_of(k) * ncol), ldk<T, KC>(xc + static_cast<int64_t>(rt.ket_s[k]) * stride));
+  flush_all();
   VecK<T, KC> out;
 #pragma unroll
   for (int c = 0; c < C; ++c)
@@ -399,6 +416,12 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
       }
     }
     if (more) commit(nxt, kk + kStage);
+    if (((kk / kStage) & 7) == 7) {  // compensated FP32: close a run of at most 8 kStage terms
+#pragma unroll
+      for (int j = 0; j < kEPT; ++j)
+#pragma unroll
+        for (int v = 0; v < K; ++v) acc[j][v].flush();
+    }
   }
 
   // Fold, one vector at a time: every entry writes its sum (sign of the beta single applied)
@@ -517,9 +540,7 @@ constexpr int kSplit2 = 2, kSplit4 = 4;
 // 4e8-determinant 36-orbital workload, or for more than 45 orbitals (integral rows of more than
 // 2048 values: the large variants drop to one wave per SIMD) its choice on the 1e8-determinant
 // 48-orbital workload.
-int fused_variant(const sbdx_ctx& c, int prec, int K) {
-  const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
-  if (tuned >= 100) return tuned;  // a kernel variant, not a sub-block width
+int builtin_variant(const sbdx_ctx& c, int prec, int K) {
   if (c.L * c.L > 2048) {
     if (K == 1) return 25616;
     if (K == 2) return 25608;
@@ -528,6 +549,11 @@ int fused_variant(const sbdx_ctx& c, int prec, int K) {
   if (K == 2) return prec == 0 ? 102408 : 25616;
   if (K == 4) return prec == 0 ? 1102404 : 25608;
   return 25604;
+}
+int fused_variant(const sbdx_ctx& c, int prec, int K) {
+  const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
+  if (tuned >= 100) return tuned;  // a kernel variant, not a sub-block width
+  return builtin_variant(c, prec, K);
 }
 
 // What a K-wide build runs: the tuned variant or sub-block width, else K = 2 sub-blocks for
@@ -615,12 +641,12 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   return mt;
 }
 
-// the variant of the two-sum FP32 kernels: the autotuner's picks when two-sum was the only
-// FP32 accumulation, for up to 45 orbitals and above (the variants are tuned for the default
-// accumulation)
+// the variant of the two-sum FP32 kernels: for one vector the 256 x 32 tile (measured 273 ms
+// against 342 ms for the default accumulation's 1024 x 8 at 4e8 determinants), else the
+// built-in FP32 choice
 inline int mix_twosum_variant(const sbdx_ctx& c, int K) {
   if (K == 1) return c.L * c.L > 2048 ? 25616 : 25632;
-  return K == 8 ? 25604 : 25608;
+  return builtin_variant(c, SBDX_FP32, K);
 }
 
 template <typename T, int K, int kAcc>
@@ -671,14 +697,17 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   using S2 = integral_constant<int, 2>;
   using S4 = integral_constant<int, 4>;
   // the candidates of mix_variant_candidates(K); anything else is refused
-  if constexpr (kAcc == SBDX_ACC_TWOSUM) {
+  if constexpr (kAcc == SBDX_ACC_TWOSUM) {  // mix_twosum_variant's choices
     if constexpr (K == 1) {
+      if (var == 25632) pick(I32{}, T256{}, S1{});
+      else pick(I16{}, T256{}, S1{});
+    } else if constexpr (K == 2) {
       if (var == 25616) pick(I16{}, T256{}, S1{});
-      else pick(I32{}, T256{}, S1{});
-    } else if constexpr (K == 8) {
-      pick(I4{}, T256{}, S1{});
-    } else {
+      else pick(I8{}, T256{}, S1{});
+    } else if constexpr (K == 4) {
       pick(I8{}, T256{}, S1{});
+    } else {
+      pick(I4{}, T256{}, S1{});
     }
   } else if constexpr (K == 1) {
     if (var == 25632) pick(I32{}, T256{}, S1{});
