@@ -78,6 +78,9 @@ def test_bench_solver_variants_are_valid_options():
     from sbd_amd._lib import DavidsonOpts
     fields = {f for f, _ in DavidsonOpts._fields_}
     for name, opts in bench.SOLVERS.items():
-        assert set(opts) - {"mlp"} <= fields, name
+        # "mlp" (network file) and "fp32_acc" (sbdx_set_fp32_accumulation) are set on the
+        # engine by the benchmark, not passed as Davidson options
+        assert set(opts) - {"mlp", "fp32_acc"} <= fields, name
+        assert opts.get("fp32_acc", 0) in (0, 1), name
         if "mlp" in opts:
             assert os.path.exists(os.path.join(ROOT, opts["mlp"])), name
