@@ -131,6 +131,7 @@ __global__ __launch_bounds__(kB) void k_pairdots(const double* __restrict__ W, i
     for (int g = 0; g < G; ++g) s[g] = 0.0;
     Bins<kBins ? LQ : 1> bins;
     bins.zero();
+#pragma unroll 2
     for (int64_t col = static_cast<int64_t>(lane) * C; col < ncol; col += 64 * C) {
       const Cols<C> a = ld_cols<C>(wr + col);
 #pragma unroll
@@ -504,11 +505,11 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
     SBDX_CHECK_LAUNCH();
     k_pairdots<LQ, true, 1, 1><<<ga, kB, 0, st>>>(dW, nb, c.A.off_s.get(), c.A.ket_s.get(), c.B.str.get(), L, rs_a.get());
     k_pairdots<LQ, true, 1, 1><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_s.get(), c.B.ket_s.get(), c.A.str.get(), L, rs_b.get());
-    // the same-spin doubles: four kets per wavefront, two columns per lane where the rows hold
+    // the same-spin doubles: eight kets per wavefront, two columns per lane where the rows hold
     // whole 16-byte words
     auto doubles = [&](unsigned g, const double* w, int64_t ncol, const SpinTab& t, double* out) {
-      if (ncol % 2 == 0) k_pairdots<1, false, 4, 2><<<g, kB, 0, st>>>(w, ncol, t.off_d.get(), t.ket_d.get(), nullptr, L, out);
-      else k_pairdots<1, false, 4, 1><<<g, kB, 0, st>>>(w, ncol, t.off_d.get(), t.ket_d.get(), nullptr, L, out);
+      if (ncol % 2 == 0) k_pairdots<1, false, 8, 2><<<g, kB, 0, st>>>(w, ncol, t.off_d.get(), t.ket_d.get(), nullptr, L, out);
+      else k_pairdots<1, false, 8, 1><<<g, kB, 0, st>>>(w, ncol, t.off_d.get(), t.ket_d.get(), nullptr, L, out);
     };
     doubles(ga, dW, nb, c.A, rd_a.get());
     doubles(gb, wT.get(), na, c.B, rd_b.get());
