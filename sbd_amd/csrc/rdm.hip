@@ -305,28 +305,48 @@ struct Rdm {
   double& G(int st, int i, int j, int k, int l) { return two[st * L4 + i + L * j + L2 * k + L3 * l]; }
 };
 
-// Same-spin single pair of spin s: bra-only orbital p, ket-only q, bra string (same spin)
-// spec_same; weight w = sign * rho; wq[o] = sign * rho_q for other-spin spectators.
-// Restates OneDiffCorrelation (correlation.h:49-86) summed over the spectator spin.
-void one_diff(Rdm& r, int s, int p, int q, uint64_t spec_same, double w, const double* wq) {
-  const int t = 1 - s;
-  r.g(s, p, q) += w;
-  uint64_t o = spec_same;
-  while (o) {
-    const int j = ctz64(o);
-    o &= o - 1;
-    r.G(s + 2 * s, q, j, p, j) += w;
-    r.G(s + 2 * s, j, q, j, p) += w;
-    r.G(s + 2 * s, q, j, j, p) -= w;
-    r.G(s + 2 * s, j, q, p, j) -= w;
+// Same-spin singles of spin s, summed per orbital pair (bra-only p, ket-only q) before they are
+// expanded: w0[p, q] = sum of sign * rho, w1[p, q, j] = the same over the singles whose bra
+// string holds j (same-spin spectators), w2[p, q, j] = sum of sign * rho_j (other-spin
+// spectators j).  Adding into these L^3 arrays keeps the per-single work in cache; expand()
+// then writes OneDiffCorrelation's index patterns (correlation.h:49-86) once per (p, q, j).
+struct SinglesByPair {
+  int L;
+  std::vector<double> w0, w1, w2;
+  explicit SinglesByPair(int l)
+      : L(l), w0(static_cast<size_t>(l) * l, 0.0), w1(static_cast<size_t>(l) * l * l, 0.0),
+        w2(static_cast<size_t>(l) * l * l, 0.0) {}
+  void add(int p, int q, uint64_t spec_same, double w, const double* wq) {
+    const size_t pq = static_cast<size_t>(p) * L + q;
+    w0[pq] += w;
+    double* a1 = &w1[pq * L];
+    for (uint64_t o = spec_same; o; o &= o - 1) a1[ctz64(o)] += w;
+    double* a2 = &w2[pq * L];
+    for (int j = 0; j < L; ++j) a2[j] += wq[j];
   }
-  for (int j = 0; j < r.L; ++j) {
-    const double v = wq[j];
-    if (v == 0.0) continue;
-    r.G(s + 2 * t, q, j, p, j) += v;
-    r.G(t + 2 * s, j, q, j, p) += v;
+  void expand(Rdm& r, int s) const {
+    const int t = 1 - s;
+    for (int p = 0; p < L; ++p)
+      for (int q = 0; q < L; ++q) {
+        const size_t pq = static_cast<size_t>(p) * L + q;
+        r.g(s, p, q) += w0[pq];
+        for (int j = 0; j < L; ++j) {
+          const double w = w1[pq * L + j];
+          if (w != 0.0) {
+            r.G(s + 2 * s, q, j, p, j) += w;
+            r.G(s + 2 * s, j, q, j, p) += w;
+            r.G(s + 2 * s, q, j, j, p) -= w;
+            r.G(s + 2 * s, j, q, p, j) -= w;
+          }
+          const double v = w2[pq * L + j];
+          if (v != 0.0) {
+            r.G(s + 2 * t, q, j, p, j) += v;
+            r.G(t + 2 * s, j, q, j, p) += v;
+          }
+        }
+      }
   }
-}
+};
 
 // TwoDiffCorrelation (correlation.h:91-130) with interleaved spin-orbitals c0<c1 (bra-only)
 // and d0<d1 (ket-only); w already carries the blocked-basis sign.
@@ -566,6 +586,10 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
     const std::vector<double>& rs = spin == 0 ? hsa : hsb;
     const std::vector<double>& rd = spin == 0 ? hda : hdb;
     const int64_t nrow = spin == 0 ? na : nb;
+    SinglesByPair sp(L);
+    // doubles summed per orbital quadruple (bra-only c0 < c1, ket-only d0 < d1) first, then
+    // expanded once per quadruple
+    std::vector<double> dq(static_cast<size_t>(L4), 0.0);
     for (int64_t x = 0; x < nrow; ++x) {
       const uint64_t sx = H.str[x];
       for (int64_t k = H.off_s[x]; k < H.off_s[x + 1]; ++k) {
@@ -574,7 +598,7 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
         const double sg = (e >> 12) & 1u ? -1.0 : 1.0;
         double wq[64];
         for (int j = 0; j < L; ++j) wq[j] = sg * rs[k * (L + 1) + 1 + j];
-        one_diff(r, spin, p, q, sx, sg * rs[k * (L + 1)], wq);
+        sp.add(p, q, sx, sg * rs[k * (L + 1)], wq);
       }
       for (int64_t k = H.off_d[x]; k < H.off_d[x + 1]; ++k) {
         const uint64_t sy = H.str[H.ket_d[k]];
@@ -586,9 +610,17 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
         dbits &= dbits - 1;
         const int d1 = ctz64(dbits);
         const int sg = sign_double(sx, c0, c1, d0, d1);
-        two_diff(r, 2 * c0 + spin, 2 * c1 + spin, 2 * d0 + spin, 2 * d1 + spin, sg * rd[k]);
+        dq[((static_cast<size_t>(c0) * L + c1) * L + d0) * L + d1] += sg * rd[k];
       }
     }
+    sp.expand(r, spin);
+    for (int c0 = 0; c0 < L; ++c0)
+      for (int c1 = c0 + 1; c1 < L; ++c1)
+        for (int d0 = 0; d0 < L; ++d0)
+          for (int d1 = d0 + 1; d1 < L; ++d1) {
+            const double w = dq[((static_cast<size_t>(c0) * L + c1) * L + d0) * L + d1];
+            if (w != 0.0) two_diff(r, 2 * c0 + spin, 2 * c1 + spin, 2 * d0 + spin, 2 * d1 + spin, w);
+          }
   }
   // alpha single x beta single
   for (int a = 0; a < L; ++a)
