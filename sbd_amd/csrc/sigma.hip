@@ -5,10 +5,11 @@
 // case.  One gather then loads K contiguous values, and list decoding, integral lookups and
 // matrix-element arithmetic are shared by the K vectors.
 //
-//  * Row pass (alpha rows, task type 2 + diagonal): one workgroup per (alpha row ia, tile of
-//    256 beta columns); every wavefront covers 64 consecutive columns of the same row, so the
+//  * Row pass (alpha rows, task type 2 + diagonal): one workgroup = one wavefront per (alpha
+//    row ia, 64 C consecutive beta columns, C = 2 where a K-value unit is 8 bytes), so the
 //    row's excitation lists are wavefront-uniform (scalar loads) and every gathered operand
-//    C[ja, ib..ib+63] is one coalesced segment (512 K bytes in FP64).
+//    C[ja, ib..ib+64C-1] is one coalesced segment; the pass runs in ket blocks whose segments
+//    fit an XCD's L2.
 //  * Mixed pass (task type 0): k_sigma_mixed, described at the kernel.
 //  * Beta part (task type 1): C is transposed, the same row kernel runs with the beta tables,
 //    and the result is added back transposed.
@@ -34,10 +35,10 @@
 namespace sbdx {
 namespace {
 
-// Columns (threads) per workgroup of the row kernels: one wavefront.  Every row gathers a
-// segment of kRowTile columns from each of its ~750 ket rows, and the rows in flight on an XCD
-// reach nearly all ket rows, so the segment size sets the L2 footprint (c2, FP64: 20,000 ket
-// rows x 512 B = 10 MB at 64 columns against 40 MB at 256; the row kernels run 18% faster).
+// Lanes per workgroup of the row kernels: one wavefront.  Every row gathers a segment of
+// kRowTile x C columns from each of its ~750 ket rows, and the rows in flight on an XCD reach
+// nearly all ket rows, so the segment size sets the L2 footprint (c2, FP64: 20,000 ket rows x
+// 512 B = 10 MB at 64 columns against 40 MB at 256); the ket blocks bound it further.
 constexpr int kRowTile = 64;
 
 struct AccD {
