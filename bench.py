@@ -533,7 +533,7 @@ def main():
         out["block_sigma"] = {}
         for k in args.block:
             tuned = None
-            if args.autotune and not (prec == SBDX_FP32 and args.fp32_acc == "twosum"):
+            if args.autotune:
                 chosen, cand = eng.autotune(prec, k, reps=1)
                 tuned = {"variant": chosen, "candidates_ms": cand}
                 log(f"[bench] autotune block {k}: {tuned}")

@@ -269,8 +269,8 @@ int sbdx_autotune(sbdx_ctx *ctx, int prec, int k, int reps, int *chosen, int *ca
                   double *ms, int cap, int *ncand);
 
 /* No counterpart in the reference: the accumulation of this context's FP32 sigma builds
- * (SBDX_ACC_FP64 or SBDX_ACC_TWOSUM; SBDX_ERR_ARG otherwise).  Two-sum builds use fixed
- * kernel variants: sbdx_autotune and sbdx_set_mix_variant act on the default accumulation. */
+ * (SBDX_ACC_FP64 or SBDX_ACC_TWOSUM; SBDX_ERR_ARG otherwise).  Each accumulation keeps its own
+ * kernel-variant choice: sbdx_autotune and sbdx_set_mix_variant act on the one in effect. */
 int sbdx_set_fp32_accumulation(sbdx_ctx *ctx, int mode);
 
 /* No counterpart in the reference: pin the alpha-single x beta-single kernel variant for

@@ -417,8 +417,6 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
     need_ints(c);
     need((k == 1 || k == 2 || k == 4 || k == 8) && (prec == SBDX_FP64 || prec == SBDX_FP32) && reps >= 1,
          "bad argument (k must be 1, 2, 4 or 8)");
-    need(prec == SBDX_FP64 || c->fp32_acc == SBDX_ACC_FP64,
-         "the autotuner tunes the default FP32 accumulation; two-sum builds use fixed variants");
     const int64_t n = c->ndet();
     const size_t es = prec == SBDX_FP64 ? 8 : 4;
     sbdx::DevBuf<unsigned char> x(n * k * es), y(n * k * es);
@@ -429,7 +427,7 @@ int sbdx_autotune(sbdx_ctx* c, int prec, int k, int reps, int* chosen, int* cand
     std::vector<int> cand;
     for (int v : sbdx::mix_variant_candidates(k))
       if (sbdx::mix_variant_fits(*c, prec, k, v)) cand.push_back(v);
-    int& slot = c->mix_variant_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)];
+    int& slot = sbdx::tuned_variant(*c, prec, k);  // the context's current FP32 accumulation
     int best = 0;
     double best_ms = 0.0;
     try {
@@ -482,7 +480,7 @@ int sbdx_set_mix_variant(sbdx_ctx* c, int prec, int k, int variant) {
          "not a mixed-kernel variant for this block width");
     need(variant == 0 || sbdx::mix_variant_fits(*c, prec, k, variant),
          "this mixed-kernel variant needs more LDS than a CU has for this problem");
-    c->mix_variant_tuned[prec == SBDX_FP64 ? 0 : 1][sbdx::k_slot(k)] = variant;
+    sbdx::tuned_variant(*c, prec, k) = variant;
   });
 }
 

@@ -146,7 +146,7 @@ struct sbdx_ctx {
   sbdx::DevBuf<float> coul_a32, coul_b32;
   sbdx::Workspace ws;
   sbdx::Prof prof;
-  int mix_variant_tuned[2][4] = {}; // autotuned mixed-kernel variant [FP64|FP32][K slot]
+  int mix_variant_tuned[3][4] = {}; // autotuned mixed-kernel variant [FP64|FP32|FP32 two-sum][K slot]
   int fp32_acc = SBDX_ACC_FP64;     // accumulation of FP32 sigma builds (sbdx_set_fp32_accumulation)
   bool stored = false;             // H stored as CSR (stored.hip): sigma builds use it
   // set while sbdx_sigma_device_noalloc runs: the sigma path then takes its transposed pair
@@ -236,6 +236,15 @@ __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 
 
 // index of a block width 1, 2, 4, 8
 inline int k_slot(int k) { return k == 1 ? 0 : k == 2 ? 1 : k == 4 ? 2 : 3; }
+
+// the tuned mixed-kernel variant of a precision, block width and the context's current FP32
+// accumulation (each accumulation keeps its own choice); 0: none
+inline int& tuned_variant(sbdx_ctx& c, int prec, int k) {
+  return c.mix_variant_tuned[prec == SBDX_FP64 ? 0 : (c.fp32_acc == SBDX_ACC_TWOSUM ? 2 : 1)][k_slot(k)];
+}
+inline int tuned_variant(const sbdx_ctx& c, int prec, int k) {
+  return c.mix_variant_tuned[prec == SBDX_FP64 ? 0 : (c.fp32_acc == SBDX_ACC_TWOSUM ? 2 : 1)][k_slot(k)];
+}
 
 // ----------------------------------------------------------------------------- launchers
 

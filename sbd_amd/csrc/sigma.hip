@@ -390,7 +390,12 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
   commit(vs, 0);
   const int64_t row_bytes = nb * K * static_cast<int64_t>(sizeof(T));
   int buf = 0;
-  for (int kk = 0; kk < nk; kk += kStage, buf ^= 1) {
+  // runs of 8 kStage terms per entry; the compensated FP32 accumulation closes each run (a
+  // no-op for the other accumulators), outside the inner loop so that it is not if-converted
+  // into every stage
+  for (int kk = 0; kk < nk;) {
+  const int krun = kk + 8 * kStage < nk ? kk + 8 * kStage : nk;
+  for (; kk < krun; kk += kStage, buf ^= 1) {
     T* cur = vs + buf * kStage * L2;
     T* nxt = vs + (buf ^ 1) * kStage * L2;
     __syncthreads();  // cur is staged; nobody reads nxt any more
@@ -417,12 +422,11 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
       }
     }
     if (more) commit(nxt, kk + kStage);
-    if (((kk / kStage) & 7) == 7) {  // compensated FP32: close a run of at most 8 kStage terms
+  }
 #pragma unroll
-      for (int j = 0; j < kEPT; ++j)
+  for (int j = 0; j < kEPT; ++j)
 #pragma unroll
-        for (int v = 0; v < K; ++v) acc[j][v].flush();
-    }
+    for (int v = 0; v < K; ++v) acc[j][v].flush();
   }
 
   // Fold, one vector at a time: every entry writes its sum (sign of the beta single applied)
@@ -542,6 +546,12 @@ constexpr int kSplit2 = 2, kSplit4 = 4;
 // 2048 values: the large variants drop to one wave per SIMD) its choice on the 1e8-determinant
 // 48-orbital workload.
 int builtin_variant(const sbdx_ctx& c, int prec, int K) {
+  // compensated FP32 (three words of state per entry and vector): the autotuner's choices at
+  // 4e8 determinants, 512 x 16 for one vector and 512 x 8 x 2 stages for two
+  if (prec != 0 && c.fp32_acc == SBDX_ACC_TWOSUM && c.L * c.L <= 2048) {
+    if (K == 1) return 51216;
+    if (K == 2) return 1051208;
+  }
   if (c.L * c.L > 2048) {
     if (K == 1) return 25616;
     if (K == 2) return 25608;
@@ -552,7 +562,7 @@ int builtin_variant(const sbdx_ctx& c, int prec, int K) {
   return 25604;
 }
 int fused_variant(const sbdx_ctx& c, int prec, int K) {
-  const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
+  const int tuned = tuned_variant(c, prec, K);
   if (tuned >= 100) return tuned;  // a kernel variant, not a sub-block width
   return builtin_variant(c, prec, K);
 }
@@ -561,7 +571,7 @@ int fused_variant(const sbdx_ctx& c, int prec, int K) {
 // K = 4 and 8 up to 45 orbitals (measured faster per vector at 4e8 determinants), else the
 // fused default.
 int mix_variant(const sbdx_ctx& c, int prec, int K) {
-  const int tuned = c.mix_variant_tuned[prec == 0 ? 0 : 1][k_slot(K)];
+  const int tuned = tuned_variant(c, prec, K);
   if (tuned) return tuned;
   if (K >= 4 && c.L * c.L <= 2048) return kSplit2;
   return fused_variant(c, prec, K);
@@ -642,18 +652,9 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
   return mt;
 }
 
-// the variant of the two-sum FP32 kernels: for one vector the 256 x 32 tile (measured 273 ms
-// against 342 ms for the default accumulation's 1024 x 8 at 4e8 determinants), else the
-// built-in FP32 choice
-inline int mix_twosum_variant(const sbdx_ctx& c, int K) {
-  if (K == 1) return c.L * c.L > 2048 ? 25616 : 25632;
-  return builtin_variant(c, SBDX_FP32, K);
-}
-
 template <typename T, int K, int kAcc>
 void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
-  const int var = kAcc == SBDX_ACC_TWOSUM ? mix_twosum_variant(c, K)
-                                          : fused_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
+  const int var = fused_variant(c, std::is_same<T, double>::value ? 0 : 1, K);
   const int stages = var / 1000000 + 1, threads = (var % 1000000) / 100;
   const MixTiles& mt = mix_tiles(c, static_cast<int64_t>(var % 100) * threads);
   if (mt.ntile == 0) return;
@@ -698,19 +699,7 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   using S2 = integral_constant<int, 2>;
   using S4 = integral_constant<int, 4>;
   // the candidates of mix_variant_candidates(K); anything else is refused
-  if constexpr (kAcc == SBDX_ACC_TWOSUM) {  // mix_twosum_variant's choices
-    if constexpr (K == 1) {
-      if (var == 25632) pick(I32{}, T256{}, S1{});
-      else pick(I16{}, T256{}, S1{});
-    } else if constexpr (K == 2) {
-      if (var == 25616) pick(I16{}, T256{}, S1{});
-      else pick(I8{}, T256{}, S1{});
-    } else if constexpr (K == 4) {
-      pick(I8{}, T256{}, S1{});
-    } else {
-      pick(I4{}, T256{}, S1{});
-    }
-  } else if constexpr (K == 1) {
+  if constexpr (K == 1) {
     if (var == 25632) pick(I32{}, T256{}, S1{});
     else if (var == 25616) pick(I16{}, T256{}, S1{});
     else if (var == 51216) pick(I16{}, T512{}, S1{});

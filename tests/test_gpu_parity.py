@@ -152,9 +152,18 @@ def test_fp32_twosum_accumulation(pair, k):
         for v in range(k):
             rel = np.max(np.abs(Y[:, v] - refs[v])) / np.max(np.abs(refs[v]))
             assert rel <= 2e-6, (mode, v, rel)
+    # each accumulation keeps its own tuned variant: tune under two-sum, check its choice,
+    # and the default accumulation still builds with its own
     eng.set_fp32_accumulation(1)
-    with pytest.raises(SbdxError):
-        eng.autotune(1, k, reps=1)  # the tuner tunes the default accumulation only
+    chosen, cand = eng.autotune(1, k, reps=1)
+    assert chosen in cand
+    dy = torch.empty_like(dx)
+    eng.sigma_block_device(1, k, dx.data_ptr(), dy.data_ptr())
+    Y = dy.cpu().numpy().astype(np.float64).reshape(so.ndet, k)
+    for v in range(k):
+        rel = np.max(np.abs(Y[:, v] - refs[v])) / np.max(np.abs(refs[v]))
+        assert rel <= 2e-6, ("two-sum autotuned", v, rel)
+    eng.set_mix_variant(1, k, 0)
     eng.set_fp32_accumulation(0)
     with pytest.raises(SbdxError):
         eng.set_fp32_accumulation(2)
