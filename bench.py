@@ -67,6 +67,9 @@ SOLVERS = {
                                   mlp="sbd_amd/weights/c2_family.npy"),
     "mixed_block2_nb24_mlp": dict(method=1, precision=1, block=2, num_block=24, preconditioner=1,
                                   mlp="sbd_amd/weights/c2_family.npy"),
+    # the network only after two diagonal restart cycles
+    "mixed_block2_nb24_mlp_w2": dict(method=1, precision=1, block=2, num_block=24, preconditioner=1,
+                                     precond_warmup=2, mlp="sbd_amd/weights/c2_family.npy"),
     # eight lowest roots in one block of eight
     "mixed_roots8": dict(method=1, precision=1, nroots=8, num_block=4),
     # the same with 5 blocks (fits 288 GB since blocks of 8 are built as two-vector sub-blocks)
