@@ -390,11 +390,12 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
   commit(vs, 0);
   const int64_t row_bytes = nb * K * static_cast<int64_t>(sizeof(T));
   int buf = 0;
-  // runs of 8 kStage terms per entry; the compensated FP32 accumulation closes each run (a
-  // no-op for the other accumulators), outside the inner loop so that it is not if-converted
-  // into every stage
+  // runs of 16 alpha singles (16 terms per entry and vector); the compensated FP32 accumulation
+  // closes each run (a no-op for the other accumulators), outside the inner loop so that it is
+  // not if-converted into every stage
+  static_assert(16 % kStage == 0, "a run holds whole stages");
   for (int kk = 0; kk < nk;) {
-  const int krun = kk + 8 * kStage < nk ? kk + 8 * kStage : nk;
+  const int krun = kk + 16 < nk ? kk + 16 : nk;
   for (; kk < krun; kk += kStage, buf ^= 1) {
     T* cur = vs + buf * kStage * L2;
     T* nxt = vs + (buf ^ 1) * kStage * L2;
