@@ -12,7 +12,6 @@
 #include "common.h"
 #include "sbdx.h"
 
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -152,6 +151,68 @@ __global__ __launch_bounds__(64 * kWaves) void k_spmv(const int64_t* __restrict_
   }
 }
 
+// Exclusive prefix sum of n int64 values: each workgroup scans 1024 of them (wavefront scans
+// of per-thread sums, then the four wavefronts' totals), the workgroups' totals are scanned on
+// the host, and a second pass adds each workgroup's offset.  Integer sums: exact.
+constexpr int kScanItems = 4;
+constexpr int kScanBlock = 256 * kScanItems;
+
+__global__ __launch_bounds__(256) void k_scan_local(const int64_t* __restrict__ in, int64_t* __restrict__ out,
+                                                    int64_t n, int64_t* __restrict__ totals) {
+  __shared__ int64_t wave_total[4];
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanBlock + static_cast<int64_t>(threadIdx.x) * kScanItems;
+  int64_t v[kScanItems];
+  int64_t t = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = base + i < n ? in[base + i] : 0;
+    t += v[i];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int64_t x = t;  // inclusive scan of the per-thread sums across the wavefront
+  for (int d = 1; d < 64; d <<= 1) {
+    const int64_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wave_total[wv] = x;
+  __syncthreads();
+  int64_t woff = 0;
+  for (int w = 0; w < wv; ++w) woff += wave_total[w];
+  int64_t run = woff + x - t;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+  if (threadIdx.x == 255) totals[blockIdx.x] = woff + x;
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int64_t* __restrict__ out, int64_t n,
+                                                  const int64_t* __restrict__ offsets) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) out[i] += offsets[i / kScanBlock];
+}
+
+void exclusive_scan(const int64_t* in, int64_t* out, int64_t n, hipStream_t st) {
+  const int64_t nblk = cdiv(n, kScanBlock);
+  DevBuf<int64_t> totals(nblk);
+  k_scan_local<<<static_cast<unsigned>(nblk), 256, 0, st>>>(in, out, n, totals.get());
+  SBDX_CHECK_LAUNCH();
+  std::vector<int64_t> h(nblk);
+  SBDX_HIP(hipMemcpyAsync(h.data(), totals.get(), nblk * 8, hipMemcpyDeviceToHost, st));
+  SBDX_HIP(hipStreamSynchronize(st));
+  int64_t run = 0;
+  for (int64_t b = 0; b < nblk; ++b) {
+    const int64_t t = h[b];
+    h[b] = run;
+    run += t;
+  }
+  SBDX_HIP(hipMemcpyAsync(totals.get(), h.data(), nblk * 8, hipMemcpyHostToDevice, st));
+  k_scan_add<<<static_cast<unsigned>(cdiv(n, 256)), 256, 0, st>>>(out, n, totals.get());
+  SBDX_CHECK_LAUNCH();
+  SBDX_HIP(hipStreamSynchronize(st));  // the host buffer h is read by the copy above
+}
+
 Lists lists(const SpinTab& t) {
   return Lists{t.off_s.get(), t.ket_s.get(), t.exc_s.get(), t.val_s.get(), t.off_d.get(), t.ket_d.get(), t.val_d.get(),
                t.str.get()};
@@ -168,10 +229,7 @@ int64_t store_hamiltonian(sbdx_ctx& c) {
   const Lists A = lists(c.A), B = lists(c.B);
   k_csr_len<<<static_cast<unsigned>(cdiv(n + 1, 256)), 256, 0, st>>>(A, B, c.na, c.nb, len.get());
   SBDX_CHECK_LAUNCH();
-  size_t tmp_bytes = 0;
-  SBDX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, len.get(), rowptr.get(), static_cast<int>(n + 1), st));
-  DevBuf<unsigned char> tmp(tmp_bytes);
-  SBDX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.get(), tmp_bytes, len.get(), rowptr.get(), static_cast<int>(n + 1), st));
+  exclusive_scan(len.get(), rowptr.get(), n + 1, st);
   int64_t nnz = 0;
   SBDX_HIP(hipMemcpyAsync(&nnz, rowptr.get() + n, 8, hipMemcpyDeviceToHost, st));
   SBDX_HIP(hipStreamSynchronize(st));
