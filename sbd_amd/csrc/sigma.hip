@@ -395,39 +395,39 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
   // not if-converted into every stage
   static_assert(16 % kStage == 0, "a run holds whole stages");
   for (int kk = 0; kk < nk;) {
-  const int krun = kk + 16 < nk ? kk + 16 : nk;
-  for (; kk < krun; kk += kStage, buf ^= 1) {
-    T* cur = vs + buf * kStage * L2;
-    T* nxt = vs + (buf ^ 1) * kStage * L2;
-    __syncthreads();  // cur is staged; nobody reads nxt any more
-    const bool more = kk + kStage < nk;
-    if (more) fetch(kk + kStage);
-    VecK<T, K> xv[kStage][kEPT];
+    const int krun = kk + 16 < nk ? kk + 16 : nk;
+    for (; kk < krun; kk += kStage, buf ^= 1) {
+      T* cur = vs + buf * kStage * L2;
+      T* nxt = vs + (buf ^ 1) * kStage * L2;
+      __syncthreads();  // cur is staged; nobody reads nxt any more
+      const bool more = kk + kStage < nk;
+      if (more) fetch(kk + kStage);
+      VecK<T, K> xv[kStage][kEPT];
 #pragma unroll
-    for (int q = 0; q < kStage; ++q) {
-      if (kk + q >= nk) break;
-      const uint32_t ja = __builtin_amdgcn_readfirstlane(s_ket[kk + q]);
-      const __amdgpu_buffer_rsrc_t xr =
-          buf_rsrc(reinterpret_cast<const unsigned char*>(x) + ja * row_bytes, static_cast<uint32_t>(row_bytes));
+      for (int q = 0; q < kStage; ++q) {
+        if (kk + q >= nk) break;
+        const uint32_t ja = __builtin_amdgcn_readfirstlane(s_ket[kk + q]);
+        const __amdgpu_buffer_rsrc_t xr =
+            buf_rsrc(reinterpret_cast<const unsigned char*>(x) + ja * row_bytes, static_cast<uint32_t>(row_bytes));
 #pragma unroll
-      for (int j = 0; j < kEPT; ++j) xv[q][j] = ldk_buf<T, K>(xr, joff[j]);
-    }
-#pragma unroll
-    for (int q = 0; q < kStage; ++q) {
-      if (kk + q >= nk) break;
-#pragma unroll
-      for (int j = 0; j < kEPT; ++j) {
-        const T w = cur[q * L2 + ((meta[j] >> 16) & 0xfff)];
-#pragma unroll
-        for (int v = 0; v < K; ++v) acc[j][v].fma(w, xv[q][j].v[v]);
+        for (int j = 0; j < kEPT; ++j) xv[q][j] = ldk_buf<T, K>(xr, joff[j]);
       }
+#pragma unroll
+      for (int q = 0; q < kStage; ++q) {
+        if (kk + q >= nk) break;
+#pragma unroll
+        for (int j = 0; j < kEPT; ++j) {
+          const T w = cur[q * L2 + ((meta[j] >> 16) & 0xfff)];
+#pragma unroll
+          for (int v = 0; v < K; ++v) acc[j][v].fma(w, xv[q][j].v[v]);
+        }
+      }
+      if (more) commit(nxt, kk + kStage);
     }
-    if (more) commit(nxt, kk + kStage);
-  }
 #pragma unroll
-  for (int j = 0; j < kEPT; ++j)
+    for (int j = 0; j < kEPT; ++j)
 #pragma unroll
-    for (int v = 0; v < K; ++v) acc[j][v].flush();
+      for (int v = 0; v < K; ++v) acc[j][v].flush();
   }
 
   // Fold, one vector at a time: every entry writes its sum (sign of the beta single applied)
