@@ -14,6 +14,7 @@
 //   from_string, makestring,       framework/bit_manipulation.h:135-172, :324-330
 //   sort_bitarray, LoadAlphaDets   chemistry/basic/makedeterminants.h:160-185 (.txt and .bin)
 //   DecodeAlphaDets                chemistry/basic/makedeterminants.h:63-108
+//   LoadFromAlphaDets              chemistry/basic/makedeterminants.h:110-149 (one rank)
 //   change_bitlength, get_extension framework/bit_manipulation.h:863-934, :1017-1038
 //   TaskHelpers, TaskCommunicator, chemistry/tpb/helper.h:12-36, :184-218, :432-574, :650-953
 //   MakeHelpers, RemakeHelpers, FreeHelpers
@@ -360,6 +361,27 @@ struct Engine {
 };
 
 }  // namespace amd_detail
+
+// Configurations of the generic operator layer from a binary determinant file
+// (makedeterminants.h:110-149): the n strings of the file (in file order, DecodeAlphaDets)
+// serve as both spins, and configuration Na * ib + ia holds the 2 norb bits of string ia
+// followed by those of string ib, bit_length bits per word.  One rank: every pair.
+inline void LoadFromAlphaDets(const std::string& filename, std::vector<std::vector<size_t>>& config,
+                              size_t norb, size_t bit_length, MPI_Comm comm) {
+  amd_detail::require_single(comm);
+  const std::vector<std::vector<size_t>> dets = DecodeAlphaDets(filename, norb);
+  const size_t n = dets.size();
+  config.assign(n * n, std::vector<size_t>());
+  std::vector<size_t> bits(2 * norb);
+  for (size_t ib = 0; ib < n; ++ib)
+    for (size_t ia = 0; ia < n; ++ia) {
+      std::copy(dets[ia].begin(), dets[ia].end(), bits.begin());
+      std::copy(dets[ib].begin(), dets[ib].end(), bits.begin() + norb);
+      std::vector<size_t> w = bits;
+      change_bitlength(1, w, bit_length);
+      config[n * ib + ia] = std::move(w);
+    }
+}
 
 // Solver used by the drop-in Davidson below; no counterpart in the reference, whose algorithm
 // the defaults keep.  generate_sbd_data also reads --amd_solver block|reference,

@@ -4,6 +4,9 @@
 // Test program: tests/test_shim_io.py compares its files with the reference's own.
 //
 //   shim_io <adets file> <bdets file> <norb> <bit_length> <out prefix> [vector file]
+//   shim_io configs <dets .bin file> <norb> <bit_length> <out prefix>
+//       sbd::LoadFromAlphaDets: <out>.config.u64 (the configurations' words, flat) and the
+//       word count per configuration in the printed line
 #include <cstdio>
 #include <fstream>
 #include <string>
@@ -14,8 +17,18 @@
 int main(int argc, char** argv) {
   int provided;
   MPI_Init_thread(&argc, &argv, MPI_THREAD_FUNNELED, &provided);
+  if (argc == 6 && std::string(argv[1]) == "configs") {
+    std::vector<std::vector<size_t>> config;
+    sbd::LoadFromAlphaDets(argv[2], config, std::stoul(argv[3]), std::stoul(argv[4]), MPI_COMM_WORLD);
+    std::ofstream f(std::string(argv[5]) + ".config.u64", std::ios::binary);
+    for (const auto& v : config) f.write(reinterpret_cast<const char*>(v.data()), sizeof(size_t) * v.size());
+    std::printf("{\"op\":\"configs\",\"n\":%zu,\"words\":%zu}\n", config.size(),
+                config.empty() ? size_t(0) : config[0].size());
+    MPI_Finalize();
+    return 0;
+  }
   if (argc < 6) {
-    std::fprintf(stderr, "usage: shim_io <adets> <bdets> <norb> <bit_length> <out prefix> [vector]\n");
+    std::fprintf(stderr, "usage: shim_io <adets> <bdets> <norb> <bit_length> <out prefix> [vector] | configs <bin> <norb> <bit_length> <out>\n");
     return 2;
   }
   const size_t norb = std::stoul(argv[3]), bit_length = std::stoul(argv[4]);

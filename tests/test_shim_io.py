@@ -145,3 +145,30 @@ def test_files_identical_to_the_compiled_reference(shim_io, tmp_path, norb, nel,
     assert r.returncode == 0, r.stderr[-2000:]
     for ext in (".adet.u64", ".bdet.u64", ".wf.bin", ".wf.txt"):
         assert open(mine + ext, "rb").read() == open(ref + ext, "rb").read(), ext
+
+
+@pytest.mark.parametrize("norb,bl", [(6, 20), (12, 7), (36, 20), (40, 64)])
+def test_load_from_alpha_dets_configurations(shim_io, tmp_path, norb, bl):
+    """LoadFromAlphaDets (makedeterminants.h:110-149, one rank): the file's strings in file
+    order (no sorting, duplicates kept) serve as both spins; configuration n * ib + ia holds
+    string ia's norb bits then string ib's, repacked at bl bits per word.  Expected words are
+    built here from the integers written to the file."""
+    rng = np.random.default_rng(norb)
+    vals = [int(v) for v in rng.integers(0, 1 << min(norb, 62), size=9)]
+    vals.append(vals[2])  # a duplicate stays
+    # a one-byte tail is a whole record when records are one byte long
+    write_bin(tmp_path / "d.bin", vals, norb, pad_bits=True, tail=norb > 8)
+    out = str(tmp_path / "cfg")
+    r = subprocess.run([shim_io, "configs", str(tmp_path / "d.bin"), str(norb), str(bl), out],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr[-2000:]
+    n = len(vals)
+    nw = (2 * norb + bl - 1) // bl
+    want = []
+    for ib in range(n):
+        for ia in range(n):
+            full = vals[ia] | (vals[ib] << norb)
+            want += [(full >> (bl * w)) & ((1 << bl) - 1) for w in range(nw)]
+    got = np.fromfile(out + ".config.u64", dtype=np.uint64)
+    assert got.size == n * n * nw
+    assert np.array_equal(got, np.array(want, dtype=np.uint64))
