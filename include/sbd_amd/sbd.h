@@ -330,12 +330,20 @@ inline void require_single(MPI_Comm c) {
   if (s != 1) throw std::invalid_argument("sbd_amd: the wavefunction lives on one GPU; communicators must have size 1");
 }
 
+// 64-bit words per packed string: one up to 64 orbitals, two up to 128
+inline int words_for(size_t L) {
+  if (L > 128) throw std::invalid_argument("sbd_amd: at most 128 orbitals");
+  return L > 64 ? 2 : 1;
+}
+
+// bit_length-bit words -> `words_for(L)` uint64 words per string, least significant first
+// (the semantics of change_bitlength, framework/bit_manipulation.h:863-893)
 inline std::vector<uint64_t> pack(const std::vector<std::vector<size_t>>& d, size_t bit_length, size_t L) {
-  if (L > 64) throw std::invalid_argument("sbd_amd: at most 64 orbitals");
-  std::vector<uint64_t> out(d.size(), 0);
+  const int W = words_for(L);
+  std::vector<uint64_t> out(d.size() * W, 0);
   for (size_t k = 0; k < d.size(); ++k)
     for (size_t p = 0; p < L; ++p)
-      if ((d[k][p / bit_length] >> (p % bit_length)) & 1u) out[k] |= uint64_t(1) << p;
+      if ((d[k][p / bit_length] >> (p % bit_length)) & 1u) out[k * W + p / 64] |= uint64_t(1) << (p % 64);
   return out;
 }
 
@@ -346,8 +354,11 @@ struct Engine {
   double I0 = 0.0;
   std::vector<double> I1, I2;
   bool have = false;
-  Engine(size_t norb, const std::vector<uint64_t>& a, const std::vector<uint64_t>& b) : L(norb), na(a.size()), nb(b.size()) {
-    check(sbdx_create(static_cast<int>(norb), static_cast<int64_t>(na), a.data(), static_cast<int64_t>(nb), b.data(), &ctx));
+  // a, b: packed strings (pack), words_for(norb) words each
+  Engine(size_t norb, const std::vector<uint64_t>& a, const std::vector<uint64_t>& b)
+      : L(norb), na(a.size() / words_for(norb)), nb(b.size() / words_for(norb)) {
+    check(sbdx_create_words(static_cast<int>(norb), words_for(norb), static_cast<int64_t>(na), a.data(),
+                            static_cast<int64_t>(nb), b.data(), &ctx));
   }
   ~Engine() { sbdx_destroy(ctx); }
   void ensure(double i0, const oneInt<double>& i1, const twoInt<double>& i2) {

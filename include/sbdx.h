@@ -10,11 +10,13 @@
  * Vectors are determinant-indexed, alpha-major: W[ia * n_beta + ib] (the reference's layout,
  * chemistry/tpb/mult.h:503-504, qcham.h:359).  Strings are uint64 bit masks, bit p = orbital
  * p, i.e. the reference's `bit_length`-bit words concatenated (framework/bit_manipulation.h:
- * 863-893 change_bitlength to 64); norb <= 64.
+ * 863-893 change_bitlength to 64): one word each for norb <= 64 (sbdx_create), `words` words
+ * each, least significant first, for norb <= 128 (sbdx_create_words).
  *
  * Entry points and the reference interface each one stands in for (file:line in r-ccs-cms/sbd
  * include/sbd/):
  *   sbdx_create            MakeHelpers / RemakeHelpers    chemistry/tpb/helper.h:432, :650
+ *   sbdx_create_words      the same, strings of any bit_length up to 128 orbitals
  *   sbdx_set_integrals     SetupIntegrals (consumer side) chemistry/basic/makeintegrals.h:10
  *   sbdx_get_excitations   TaskHelpers lists              chemistry/tpb/helper.h:12-36, :132-182
  *   sbdx_hdiag             makeQChamDiagTerms             chemistry/tpb/qcham.h:314-366
@@ -36,7 +38,7 @@ extern "C" {
 
 typedef enum {
   SBDX_OK = 0,
-  SBDX_ERR_ARG = -1,       /* invalid argument (sizes, null pointers, norb > 64, ...) */
+  SBDX_ERR_ARG = -1,       /* invalid argument (sizes, null pointers, norb > 128, ...) */
   SBDX_ERR_STATE = -2,     /* call out of order (e.g. sigma before set_integrals) */
   SBDX_ERR_DEVICE = -3,    /* HIP runtime error (allocation, launch) */
   SBDX_ERR_NOCONV = -4     /* not an error of the call: Davidson stopped before tolerance */
@@ -59,6 +61,14 @@ typedef struct sbdx_ctx sbdx_ctx;
  * order; each string's index is its row/column in W).  Builds the excitation lists on the GPU. */
 int sbdx_create(int norb, int64_t n_alpha, const uint64_t *alpha, int64_t n_beta,
                 const uint64_t *beta, sbdx_ctx **out);
+
+/* sbdx_create for strings of `words` (1 or 2) uint64 words each, least significant word first
+ * (orbital p = bit p % 64 of word p / 64), norb <= 64 words <= 128; bits at or above norb must
+ * be clear.  Above 64 orbitals the mixed pass reads its integral rows from global memory
+ * (they no longer fit the LDS twice), runs its 256-thread one-stage variants only, and the
+ * learned preconditioner is not available (its inputs hold 64 orbitals per spin). */
+int sbdx_create_words(int norb, int words, int64_t n_alpha, const uint64_t *alpha, int64_t n_beta,
+                      const uint64_t *beta, sbdx_ctx **out);
 
 /* Integrals in the reference's storage: I1 = oneInt::store, (2 norb)^2 spin-orbital entries,
  * I1[i*2L+j] (even index = alpha, odd = beta); I2 = twoInt::store, packed spatial (pq|rs) of

@@ -211,9 +211,11 @@ def write_instance(path, norb, nel_a, nel_b, alpha, beta, vals, idx):
     beta = np.ascontiguousarray(beta, np.uint64)
     vals = np.ascontiguousarray(vals, np.float64)
     idx = np.ascontiguousarray(idx, np.int32).reshape(-1)
+    # version 2: (n, words) string arrays, least significant word first
+    ver, words = (1, 0) if alpha.ndim == 1 else (2, alpha.shape[1])
     with open(path, "wb") as f:
         f.write(b"SBDI")
-        f.write(struct.pack("<I4i3q", 1, norb, nel_a, nel_b, 0, alpha.size, beta.size, vals.size))
+        f.write(struct.pack("<I4i3q", ver, norb, nel_a, nel_b, words, alpha.shape[0], beta.shape[0], vals.size))
         f.write(alpha.tobytes())
         f.write(beta.tobytes())
         f.write(vals.tobytes())

@@ -9,8 +9,9 @@
 // the integrals handed over in memory (no 6-digit FCIDUMP text broadcast, fcidump.h:166).
 //
 // Instance file (little endian):
-//   char[4] "SBDI", u32 version(1), i32 L, i32 nel_a, i32 nel_b, i32 pad,
-//   i64 na, i64 nb, i64 nent, u64 alpha[na], u64 beta[nb], f64 val[nent], i32 idx[4*nent]
+//   char[4] "SBDI", u32 version, i32 L, i32 nel_a, i32 nel_b, i32 words (version 1: 0, one word),
+//   i64 na, i64 nb, i64 nent, u64 alpha[na * words], u64 beta[nb * words], f64 val[nent],
+//   i32 idx[4*nent]; a string is `words` uint64 words, least significant first
 #include <cassert>
 #include <chrono>
 #include <cmath>
@@ -36,7 +37,7 @@
 namespace {
 
 struct Inst {
-  int L = 0, nel_a = 0, nel_b = 0;
+  int L = 0, nel_a = 0, nel_b = 0, W = 1;
   std::vector<uint64_t> a, b;
   std::vector<double> val;
   std::vector<int32_t> idx;
@@ -53,30 +54,31 @@ Inst read_instance(const std::string& path) {
   f.read(reinterpret_cast<char*>(&ver), 4);
   f.read(reinterpret_cast<char*>(hdr), 16);
   f.read(reinterpret_cast<char*>(n3), 24);
-  if (std::memcmp(magic, "SBDI", 4) != 0 || ver != 1) throw std::runtime_error("bad instance");
+  if (std::memcmp(magic, "SBDI", 4) != 0 || (ver != 1 && ver != 2)) throw std::runtime_error("bad instance");
   Inst in;
   in.L = hdr[0];
   in.nel_a = hdr[1];
   in.nel_b = hdr[2];
-  in.a.resize(n3[0]);
-  in.b.resize(n3[1]);
+  in.W = ver == 2 ? hdr[3] : 1;
+  in.a.resize(n3[0] * in.W);
+  in.b.resize(n3[1] * in.W);
   in.val.resize(n3[2]);
   in.idx.resize(4 * n3[2]);
-  f.read(reinterpret_cast<char*>(in.a.data()), 8 * n3[0]);
-  f.read(reinterpret_cast<char*>(in.b.data()), 8 * n3[1]);
+  f.read(reinterpret_cast<char*>(in.a.data()), 8 * n3[0] * in.W);
+  f.read(reinterpret_cast<char*>(in.b.data()), 8 * n3[1] * in.W);
   f.read(reinterpret_cast<char*>(in.val.data()), 8 * n3[2]);
   f.read(reinterpret_cast<char*>(in.idx.data()), 16 * n3[2]);
   if (!f) throw std::runtime_error("short instance file");
   return in;
 }
 
-// uint64 mask (bit p = orbital p) -> reference words of bit_length bits
-std::vector<std::vector<size_t>> to_words(const std::vector<uint64_t>& s, int L, size_t bl) {
+// W-word uint64 masks (bit p = orbital p) -> reference words of bit_length bits
+std::vector<std::vector<size_t>> to_words(const std::vector<uint64_t>& s, int L, size_t bl, int W) {
   size_t nw = (L + bl - 1) / bl;
-  std::vector<std::vector<size_t>> out(s.size(), std::vector<size_t>(nw, 0));
-  for (size_t k = 0; k < s.size(); ++k)
+  std::vector<std::vector<size_t>> out(s.size() / W, std::vector<size_t>(nw, 0));
+  for (size_t k = 0; k < out.size(); ++k)
     for (int p = 0; p < L; ++p)
-      if ((s[k] >> p) & 1u) out[k][p / bl] |= size_t(1) << (p % bl);
+      if ((s[k * W + p / 64] >> (p % 64)) & 1u) out[k][p / bl] |= size_t(1) << (p % bl);
   return out;
 }
 
@@ -191,8 +193,8 @@ int main(int argc, char** argv) {
   for (size_t e = 0; e < inst.val.size(); ++e)
     fcidump.integrals.emplace_back(inst.val[e], inst.idx[4 * e], inst.idx[4 * e + 1],
                                    inst.idx[4 * e + 2], inst.idx[4 * e + 3]);
-  auto adet = to_words(inst.a, inst.L, bit_length);
-  auto bdet = to_words(inst.b, inst.L, bit_length);
+  auto adet = to_words(inst.a, inst.L, bit_length, inst.W);
+  auto bdet = to_words(inst.b, inst.L, bit_length, inst.W);
 
   int L, N;
   double I0;
@@ -211,7 +213,7 @@ int main(int argc, char** argv) {
 
   std::vector<double> hii;
   sbd::makeQChamDiagTerms(adet, bdet, bit_length, L, helper, I0, I1, I2, hii, h_comm, b_comm, t_comm);
-  const size_t ndet = inst.a.size() * inst.b.size();
+  const size_t ndet = adet.size() * bdet.size();
 
   std::vector<std::string> ops;
   {

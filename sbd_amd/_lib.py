@@ -16,7 +16,7 @@ SBDX_FP64 = 0
 SBDX_FP32 = 1
 
 EXPORTS = [
-    "sbdx_create", "sbdx_set_integrals", "sbdx_excitation_counts", "sbdx_get_excitations",
+    "sbdx_create", "sbdx_create_words", "sbdx_set_integrals", "sbdx_excitation_counts", "sbdx_get_excitations",
     "sbdx_ndet", "sbdx_hdiag", "sbdx_sigma", "sbdx_sigma_device", "sbdx_sigma_block_device",
     "sbdx_davidson_defaults",
     "sbdx_davidson", "sbdx_davidson_device", "sbdx_rdm", "sbdx_occupation", "sbdx_profile",
@@ -74,6 +74,7 @@ def lib():
     L = C.CDLL(LIB_PATH)
     vp = C.c_void_p
     L.sbdx_create.argtypes = [C.c_int, C.c_int64, u64p, C.c_int64, u64p, C.POINTER(vp)]
+    L.sbdx_create_words.argtypes = [C.c_int, C.c_int, C.c_int64, u64p, C.c_int64, u64p, C.POINTER(vp)]
     L.sbdx_set_integrals.argtypes = [vp, C.c_double, f64p, f64p, C.c_int64]
     L.sbdx_excitation_counts.argtypes = [vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.sbdx_get_excitations.argtypes = [vp, C.c_int, i64p, i64p, i64p, i64p]
@@ -195,12 +196,21 @@ class Engine:
     """One selected-basis problem resident on the current GPU (an ``sbdx_ctx``)."""
 
     def __init__(self, norb, alpha, beta, I0=None, I1=None, I2=None):
+        """alpha, beta: uint64 strings, shape (n,) up to 64 orbitals or (n, words) with the least
+        significant word first (sbdx_create_words, up to 128 orbitals)."""
         self.norb = int(norb)
         self.alpha = np.ascontiguousarray(alpha, dtype=np.uint64)
         self.beta = np.ascontiguousarray(beta, dtype=np.uint64)
         h = C.c_void_p()
-        check(lib().sbdx_create(self.norb, self.alpha.size, self.alpha, self.beta.size, self.beta,
-                                C.byref(h)))
+        if self.alpha.ndim == 1 and self.beta.ndim == 1:
+            check(lib().sbdx_create(self.norb, self.alpha.size, self.alpha, self.beta.size, self.beta,
+                                    C.byref(h)))
+        else:
+            words = self.alpha.shape[1]
+            if self.alpha.ndim != 2 or self.beta.ndim != 2 or self.beta.shape[1] != words:
+                raise ValueError("alpha and beta must both be (n, words) arrays")
+            check(lib().sbdx_create_words(self.norb, words, self.alpha.shape[0], self.alpha,
+                                          self.beta.shape[0], self.beta, C.byref(h)))
         self._h = h
         if I1 is not None:
             self.set_integrals(I0, I1, I2)
@@ -234,7 +244,7 @@ class Engine:
         return ns.value, nd.value
 
     def excitations(self, spin):
-        n = self.alpha.size if spin == 0 else self.beta.size
+        n = self.alpha.shape[0] if spin == 0 else self.beta.shape[0]
         ns, nd = self.excitation_counts(spin)
         off_s = np.zeros(n + 1, np.int64)
         off_d = np.zeros(n + 1, np.int64)

@@ -82,33 +82,50 @@ void download_user(sbdx_ctx& c, const double* dev, double* host, sbdx::DevBuf<do
 
 extern "C" {
 
-int sbdx_create(int norb, int64_t n_alpha, const uint64_t* alpha, int64_t n_beta,
-                const uint64_t* beta, sbdx_ctx** out) {
+int sbdx_create_words(int norb, int words, int64_t n_alpha, const uint64_t* alpha, int64_t n_beta,
+                      const uint64_t* beta, sbdx_ctx** out) {
   return guarded([&] {
     need(out != nullptr, "out is null");
     *out = nullptr;
-    need(norb >= 1 && norb <= 64, "norb must be in [1, 64]");
+    need(words == 1 || words == 2, "words must be 1 or 2");
+    need(norb >= 1 && norb <= 64 * words, "norb must be in [1, 64 words]");
     need(n_alpha >= 1 && n_beta >= 1, "empty string set");
     need(n_alpha < (int64_t(1) << 32) && n_beta < (int64_t(1) << 32), "too many strings");
     need(alpha && beta, "null string table");
-    const uint64_t mask = norb >= 64 ? ~0ull : ((1ull << norb) - 1ull);
-    for (int64_t i = 0; i < n_alpha; ++i) need((alpha[i] & ~mask) == 0, "alpha string has bits >= norb");
-    for (int64_t i = 0; i < n_beta; ++i) need((beta[i] & ~mask) == 0, "beta string has bits >= norb");
-    // every string of one spin must hold the same electron count: the excitation tables
-    // decode a popcount-2 (-4) difference as exactly one (two) electron(s) moved
-    for (int64_t i = 1; i < n_alpha; ++i)
-      need(sbdx::popc64(alpha[i]) == sbdx::popc64(alpha[0]), "alpha strings differ in electron count");
-    for (int64_t i = 1; i < n_beta; ++i)
-      need(sbdx::popc64(beta[i]) == sbdx::popc64(beta[0]), "beta strings differ in electron count");
+    // the context keeps W = 1 word per string up to 64 orbitals and 2 above
+    const int W = norb > 64 ? 2 : 1;
+    std::vector<uint64_t> sa(n_alpha * W), sb(n_beta * W);
+    auto take = [&](const uint64_t* in, int64_t n, std::vector<uint64_t>& outv, const char* bits_msg,
+                    const char* count_msg) {
+      int pop0 = 0;
+      for (int64_t i = 0; i < n; ++i) {
+        int pop = 0;
+        for (int k = 0; k < words; ++k) {
+          const int bits = std::min(64, std::max(0, norb - 64 * k));
+          const uint64_t mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1ull);
+          const uint64_t v = in[i * words + k];
+          need((v & ~mask) == 0, bits_msg);
+          if (k < W) outv[i * W + k] = v;
+          pop += sbdx::popc64(v);
+        }
+        // every string of one spin must hold the same electron count: the excitation tables
+        // decode a popcount-2 (-4) difference as exactly one (two) electron(s) moved
+        if (i == 0) pop0 = pop;
+        need(pop == pop0, count_msg);
+      }
+    };
+    take(alpha, n_alpha, sa, "alpha string has bits >= norb", "alpha strings differ in electron count");
+    take(beta, n_beta, sb, "beta string has bits >= norb", "beta strings differ in electron count");
     auto* c = new sbdx_ctx();
     try {
       SBDX_HIP(hipGetDevice(&c->device));
       (void)sbdx::noalloc_scratch(c->device);  // once per device: the no-allocation sigma's scratch
       SBDX_HIP(hipStreamCreate(&c->stream));
       c->L = norb;
+      c->W = W;
       c->na = n_alpha;
       c->nb = n_beta;
-      sbdx::build_tables(*c, alpha, beta);
+      sbdx::build_tables(*c, sa.data(), sb.data());
     } catch (...) {
       if (c->stream) (void)hipStreamDestroy(c->stream);
       delete c;
@@ -116,6 +133,12 @@ int sbdx_create(int norb, int64_t n_alpha, const uint64_t* alpha, int64_t n_beta
     }
     *out = c;
   });
+}
+
+int sbdx_create(int norb, int64_t n_alpha, const uint64_t* alpha, int64_t n_beta,
+                const uint64_t* beta, sbdx_ctx** out) {
+  if (norb > 64) return guarded([&] { need(false, "norb must be in [1, 64] (sbdx_create_words takes up to 128)"); });
+  return sbdx_create_words(norb, 1, n_alpha, alpha, n_beta, beta, out);
 }
 
 int sbdx_set_integrals(sbdx_ctx* c, double I0, const double* I1, const double* I2, int64_t n_I2) {

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace sbdx {
@@ -69,11 +70,12 @@ struct DevBuf {
 };
 
 // Excitation tables of one spin, CSR over bra strings x, kets in ascending index order.
-// exc_s packs a single excitation: bits 0-5 annihilated orbital i (occupied in the ket),
-// bits 6-11 created orbital a (occupied in the bra), bit 12 set if the blocked sign is -1.
+// exc_s packs a single excitation: bits 0-6 annihilated orbital i (occupied in the ket),
+// bits 7-13 created orbital a (occupied in the bra), bit 14 set if the blocked sign is -1
+// (exc_pack / exc_i / exc_a / exc_neg).
 struct SpinTab {
   int64_t n = 0;
-  DevBuf<uint64_t> str;
+  DevBuf<uint64_t> str;          // n strings of sbdx_ctx::W words each, least significant first
   DevBuf<int64_t> off_s, off_d;
   DevBuf<uint32_t> ket_s, ket_d, exc_s;
   DevBuf<double> val_s, val_d;   // singles: sign * (h_ai + same-spin sum); doubles: element
@@ -104,7 +106,7 @@ struct Prof {
 // cut into tiles of consecutive beta strings holding at most cap entries; inside a tile
 // entries are sorted by ket column jb.  jm = (jb, meta), meta = slot (bits 0-15: the entry's
 // index in the beta list minus the tile's first index, so each column's entries have
-// consecutive slots in the list's own order) | pb = b*L + j (16-27) | sign (bit 31).
+// consecutive slots in the list's own order) | pb = b*L + j (16-29) | sign (bit 31).
 struct MixTiles {
   int ntile = 0;
   int max_cols = 0;
@@ -128,6 +130,7 @@ struct sbdx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int L = 0;
+  int W = 1;                       // 64-bit words per string: 1 (L <= 64) or 2 (L <= 128)
   int64_t na = 0, nb = 0;
   sbdx::SpinTab A, B;
   sbdx::MixTiles mix[6];           // mixed-pass tiles, one per tile size in use
@@ -230,6 +233,120 @@ __host__ __device__ inline int det_sign(uint64_t a, uint64_t b) {
     a &= a - 1;
   }
   return (c & 1) ? -1 : 1;
+}
+
+// ----------------------------------------------------------------------------- multi-word strings
+// A string of W 64-bit words, orbital p at bit p % 64 of word p / 64 (W = 1: L <= 64, W = 2:
+// L <= 128).  The kernels that read strings take W as a template argument; with W = 1 every
+// helper below is the one-word expression above.
+template <int W>
+struct Str {
+  uint64_t w[W];
+};
+
+template <int W>
+__host__ __device__ inline Str<W> ld_str(const uint64_t* s, int64_t i) {
+  Str<W> r;
+#pragma unroll
+  for (int k = 0; k < W; ++k) r.w[k] = s[i * W + k];
+  return r;
+}
+template <int W>
+__host__ __device__ inline int popc(const Str<W>& a) {
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) c += popc64(a.w[k]);
+  return c;
+}
+template <int W>
+__host__ __device__ inline Str<W> andn(const Str<W>& a, const Str<W>& b) {  // a & ~b
+  Str<W> r;
+#pragma unroll
+  for (int k = 0; k < W; ++k) r.w[k] = a.w[k] & ~b.w[k];
+  return r;
+}
+template <int W>
+__host__ __device__ inline bool test(const Str<W>& a, int p) {
+  return (a.w[p >> 6] >> (p & 63)) & 1ull;
+}
+// lowest set orbital (the string is not empty)
+template <int W>
+__host__ __device__ inline int first(const Str<W>& a) {
+#pragma unroll
+  for (int k = 0; k < W - 1; ++k)
+    if (a.w[k]) return 64 * k + ctz64(a.w[k]);
+  return 64 * (W - 1) + ctz64(a.w[W - 1]);
+}
+// the string without its lowest set orbital
+template <int W>
+__host__ __device__ inline Str<W> drop_first(Str<W> a) {
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    if (a.w[k]) {
+      a.w[k] &= a.w[k] - 1;
+      break;
+    }
+  return a;
+}
+// f(p) for every set orbital p, ascending
+template <int W, class F>
+__host__ __device__ inline void for_each_bit(const Str<W>& a, F&& f) {
+#pragma unroll
+  for (int k = 0; k < W; ++k)
+    for (uint64_t o = a.w[k]; o; o &= o - 1) f(64 * k + ctz64(o));
+}
+// set orbitals in [lo, hi), 0 <= lo <= hi <= 64 W
+template <int W>
+__host__ __device__ inline int count_range(const Str<W>& a, int lo, int hi) {
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    const int l = lo - 64 * k, h = hi - 64 * k;
+    c += popc64(a.w[k] & mask_range(l < 0 ? 0 : (l > 64 ? 64 : l), h < 0 ? 0 : (h > 64 ? 64 : h)));
+  }
+  return c;
+}
+template <int W>
+__host__ __device__ inline int sign_single(const Str<W>& y, int i, int a) {
+  const int lo = i < a ? i : a, hi = i < a ? a : i;
+  return (count_range(y, lo + 1, hi) & 1) ? -1 : 1;
+}
+template <int W>
+__host__ __device__ inline int parity_ref(const Str<W>& y, int st, int en) {
+  const int s = (count_range(y, st, en) & 1) ? -1 : 1;
+  return test(y, st) ? -s : s;
+}
+template <int W>
+__host__ __device__ inline int sign_double(const Str<W>& y, int I, int J, int A, int B) {
+  int s = parity_ref(y, I < A ? I : A, I < A ? A : I);
+  s *= parity_ref(y, J < B ? J : B, J < B ? B : J);
+  if (A > J || B < I) s = -s;
+  return s;
+}
+template <int W>
+__host__ __device__ inline int det_sign(const Str<W>& a, const Str<W>& b) {
+  int c = 0;
+  for_each_bit(a, [&](int p) { c += count_range(b, 0, p); });
+  return (c & 1) ? -1 : 1;
+}
+
+// f(std::integral_constant<int, W>) for a context's string width
+template <class F>
+inline void with_words(int W, F&& f) {
+  if (W == 1) f(std::integral_constant<int, 1>{});
+  else f(std::integral_constant<int, 2>{});
+}
+
+// packed single excitation (SpinTab::exc_s)
+__host__ __device__ inline uint32_t exc_pack(int i, int a, bool neg) {
+  return static_cast<uint32_t>(i) | (static_cast<uint32_t>(a) << 7) | (neg ? (1u << 14) : 0u);
+}
+__host__ __device__ inline int exc_i(uint32_t e) { return static_cast<int>(e & 127u); }
+__host__ __device__ inline int exc_a(uint32_t e) { return static_cast<int>((e >> 7) & 127u); }
+__host__ __device__ inline bool exc_neg(uint32_t e) { return (e >> 14) & 1u; }
+// orbital pair a L + i of the excitation: the row of its integrals (a i | . .)
+__host__ __device__ inline uint32_t exc_pair(uint32_t e, int L) {
+  return static_cast<uint32_t>(exc_a(e)) * static_cast<uint32_t>(L) + static_cast<uint32_t>(exc_i(e));
 }
 
 __host__ __device__ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }

@@ -356,9 +356,9 @@ void jacobi_syev_host(int n, double* a, int lda, double* w) { jacobi_syev(n, a, 
 
 void init_hf(sbdx_ctx& c, double* dW, hipStream_t st) {
   // HF vector W[0] = 1 (davidson.h:101-140, init 0) expressed in the blocked basis
-  uint64_t a0 = 0, b0 = 0;
-  SBDX_HIP(hipMemcpy(&a0, c.A.str.get(), 8, hipMemcpyDeviceToHost));
-  SBDX_HIP(hipMemcpy(&b0, c.B.str.get(), 8, hipMemcpyDeviceToHost));
+  Str<2> a0{}, b0{};  // the first strings, high word zero when W = 1
+  SBDX_HIP(hipMemcpy(a0.w, c.A.str.get(), 8 * c.W, hipMemcpyDeviceToHost));
+  SBDX_HIP(hipMemcpy(b0.w, c.B.str.get(), 8 * c.W, hipMemcpyDeviceToHost));
   const double v = det_sign(a0, b0) < 0 ? -1.0 : 1.0;
   k_set_unit<<<reduce_blocks(c.ndet()), kBlock, 0, st>>>(dW, c.ndet(), v);
   SBDX_CHECK_LAUNCH();

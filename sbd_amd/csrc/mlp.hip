@@ -628,6 +628,7 @@ void mlp_set(sbdx_ctx& c, const float* params) {
     c.mlp_pb.release();
     return;
   }
+  if (c.L > 64) throw Error(SBDX_ERR_ARG, "the network's inputs hold at most 64 orbitals per spin");
   std::vector<float> w(NPARAM);
   for (int q = 0; q < NPARAM; ++q) {
     if (!std::isfinite(params[q])) throw Error(SBDX_ERR_ARG, "network parameters must be finite");
@@ -648,6 +649,7 @@ void mlp_set(sbdx_ctx& c, const float* params) {
 // dX, dXs: device vectors in the blocked basis
 int64_t mlp_record(sbdx_ctx& c, const double* dX, const double* dXs, double target, uint64_t seed,
                    sbdx_mlp_sample* out, int64_t cap, double* theta_out) {
+  if (c.L > 64) throw Error(SBDX_ERR_ARG, "the network's inputs hold at most 64 orbitals per spin");
   const int64_t n = c.ndet();
   hipStream_t st = c.stream;
   DevBuf<double> hx(n), part(4096 * 4);

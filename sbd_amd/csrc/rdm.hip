@@ -50,11 +50,13 @@ struct Bins {
 
 // One workgroup per row (4 wavefronts over the columns): rowsq[row] = sum_col W^2 and, when t
 // is given, t[row * L + q] = sum_col W^2 n_q(col).  Lane sums -> wavefront sums -> the four
-// wavefronts added in order: no atomics, the same bits on every run.
-template <int LQ>
+// wavefronts added in order: no atomics, the same bits on every run.  Strings of SW words: the
+// bins cover word `word` (orbitals 64 word + q), one launch per word.
+template <int LQ, int SW = 1>
 __global__ __launch_bounds__(kB) void k_rowstats(const double* __restrict__ W, int64_t ncol,
                                                  const uint64_t* __restrict__ col_str, int L,
-                                                 double* __restrict__ rowsq, double* __restrict__ t) {
+                                                 double* __restrict__ rowsq, double* __restrict__ t,
+                                                 int word = 0) {
   __shared__ double part[kB / 64][LQ + 1];
   const int64_t row = blockIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(kB) void k_rowstats(const double* __restrict__ W, i
     const double w = W[row * ncol + col];
     const double w2 = w * w;
     s += w2;
-    if (t) bins.add(col_str[col], w2);
+    if (t) bins.add(col_str[col * SW + word], w2);
   }
   s = wave_sum(s);
   if (lane == 0) part[wv][0] = s;
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(kB) void k_rowstats(const double* __restrict__ W, i
     double r = 0.0;
     for (int k = 0; k < kB / 64; ++k) r += part[k][threadIdx.x];
     if (threadIdx.x == 0) rowsq[row] = r;
-    else if (t && threadIdx.x - 1 < L) t[row * L + threadIdx.x - 1] = r;
+    else if (t && 64 * word + threadIdx.x - 1 < L) t[row * L + 64 * word + threadIdx.x - 1] = r;
   }
 }
 
@@ -109,12 +111,12 @@ __device__ inline Cols<C> ld_cols(const double* p) {
   return r;
 }
 
-template <int LQ, bool kBins, int G, int C>
+template <int LQ, bool kBins, int G, int C, int SW = 1>
 __global__ __launch_bounds__(kB) void k_pairdots(const double* __restrict__ W, int64_t ncol,
                                                  const int64_t* __restrict__ off,
                                                  const uint32_t* __restrict__ ket,
                                                  const uint64_t* __restrict__ col_str, int L,
-                                                 double* __restrict__ out) {
+                                                 double* __restrict__ out, int word = 0) {
   static_assert(!kBins || G == 1, "the weighted sums keep one ket's bins per lane");
   const int64_t row = blockIdx.x;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -141,7 +143,7 @@ __global__ __launch_bounds__(kB) void k_pairdots(const double* __restrict__ W, i
         for (int c = 0; c < C; ++c) {
           const double p = a.v[c] * b.v[c];
           s[g] += p;
-          if constexpr (kBins) bins.add(col_str[col + c], p);
+          if constexpr (kBins) bins.add(col_str[(col + c) * SW + word], p);
         }
       }
     }
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(kB) void k_pairdots(const double* __restrict__ W, i
 #pragma unroll
       for (int q = 0; q < (kBins ? LQ : 1); ++q) {
         const double v = wave_sum(bins.b[q]);
-        if (lane == 0 && q < L) out[k0 * stride + 1 + q] = v;
+        if (lane == 0 && 64 * word + q < L) out[k0 * stride + 1 + 64 * word + q] = v;
       }
     }
   }
@@ -275,14 +277,16 @@ std::vector<T> to_host(const DevBuf<T>& d, int64_t n) {
 }
 
 struct HostSpin {
-  std::vector<uint64_t> str;
+  int64_t n = 0;
+  std::vector<uint64_t> str;  // n strings of the context's W words
   std::vector<int64_t> off_s, off_d;
   std::vector<uint32_t> ket_s, ket_d, exc_s;
 };
 
-HostSpin host_spin(const SpinTab& t) {
+HostSpin host_spin(const SpinTab& t, int W) {
   HostSpin h;
-  h.str = to_host(t.str, t.n);
+  h.n = t.n;
+  h.str = to_host(t.str, t.n * W);
   h.off_s = to_host(t.off_s, t.n + 1);
   h.off_d = to_host(t.off_d, t.n + 1);
   h.ket_s = to_host(t.ket_s, t.nnz_s);
@@ -316,11 +320,12 @@ struct SinglesByPair {
   explicit SinglesByPair(int l)
       : L(l), w0(static_cast<size_t>(l) * l, 0.0), w1(static_cast<size_t>(l) * l * l, 0.0),
         w2(static_cast<size_t>(l) * l * l, 0.0) {}
-  void add(int p, int q, uint64_t spec_same, double w, const double* wq) {
+  template <int SW>
+  void add(int p, int q, const Str<SW>& spec_same, double w, const double* wq) {
     const size_t pq = static_cast<size_t>(p) * L + q;
     w0[pq] += w;
     double* a1 = &w1[pq * L];
-    for (uint64_t o = spec_same; o; o &= o - 1) a1[ctz64(o)] += w;
+    for_each_bit(spec_same, [&](int j) { a1[j] += w; });
     double* a2 = &w2[pq * L];
     for (int j = 0; j < L; ++j) a2[j] += wq[j];
   }
@@ -384,15 +389,15 @@ RdmTiles rdm_tiles(const HostSpin& B, int L, int max_alpha_singles) {
     uint32_t r, jb, ib, meta;
   };
   std::vector<Ent> ent;
-  const int64_t nb = static_cast<int64_t>(B.str.size());
+  const int64_t nb = B.n;
   ent.reserve(B.ket_s.size());
   for (int64_t ib = 0; ib < nb; ++ib)
     for (int64_t k = B.off_s[ib]; k < B.off_s[ib + 1]; ++k) {
       const uint32_t e = B.exc_s[k];
-      const uint32_t pb = ((e >> 6) & 63) * static_cast<uint32_t>(L) + (e & 63);
+      const uint32_t pb = exc_pair(e, L);
       const uint32_t r = pb / static_cast<uint32_t>(T.pbr);
       ent.push_back(Ent{r, B.ket_s[k], static_cast<uint32_t>(ib),
-                        (pb - r * static_cast<uint32_t>(T.pbr)) | (((e >> 12) & 1u) << 31)});
+                        (pb - r * static_cast<uint32_t>(T.pbr)) | (exc_neg(e) ? 1u << 31 : 0u)});
     }
   std::sort(ent.begin(), ent.end(), [](const Ent& u, const Ent& v) {
     return u.r != v.r ? u.r < v.r : (u.jb != v.jb ? u.jb < v.jb : u.ib < v.ib);
@@ -456,14 +461,13 @@ void rdm_mixed(sbdx_ctx& c, const double* dW, const HostSpin& A, const HostSpin&
       // the chunk's alpha singles grouped by pair pa, ascending k; ~k marks s_a = -1
       std::vector<int64_t> pa_off(L2 + 1, 0), pa_k(nk);
       for (int64_t k = k_base; k < k_base + nk; ++k) {
-        const uint32_t e = A.exc_s[k];
-        ++pa_off[((e >> 6) & 63) * L + (e & 63) + 1];
+        ++pa_off[exc_pair(A.exc_s[k], L) + 1];
       }
       for (int p = 0; p < L2; ++p) pa_off[p + 1] += pa_off[p];
       std::vector<int64_t> fill(pa_off.begin(), pa_off.end() - 1);
       for (int64_t k = k_base; k < k_base + nk; ++k) {
         const uint32_t e = A.exc_s[k];
-        pa_k[fill[((e >> 6) & 63) * L + (e & 63)]++] = (e >> 12) & 1u ? ~k : k;
+        pa_k[fill[exc_pair(e, L)]++] = exc_neg(e) ? ~k : k;
       }
       auto d_pa_off = to_dev(pa_off, st);
       auto d_pa_k = to_dev(pa_k, st);
@@ -495,14 +499,16 @@ void occupation_blocked(sbdx_ctx& c, const double* dW, double* density) {
   SBDX_CHECK_LAUNCH();
   SBDX_HIP(hipStreamSynchronize(st));
   auto ra = to_host(rowsq, na), cb = to_host(colsq, nb);
-  auto sa = to_host(c.A.str, na), sb = to_host(c.B.str, nb);
+  const int SW = c.W;
+  auto sa = to_host(c.A.str, na * SW), sb = to_host(c.B.str, nb * SW);
+  auto bit = [&](const std::vector<uint64_t>& s, int64_t x, int p) { return (s[x * SW + (p >> 6)] >> (p & 63)) & 1u; };
   for (int i = 0; i < 2 * L; ++i) density[i] = 0.0;
   for (int64_t ia = 0; ia < na; ++ia)
     for (int p = 0; p < L; ++p)
-      if ((sa[ia] >> p) & 1u) density[2 * p] += ra[ia];
+      if (bit(sa, ia, p)) density[2 * p] += ra[ia];
   for (int64_t ib = 0; ib < nb; ++ib)
     for (int p = 0; p < L; ++p)
-      if ((sb[ib] >> p) & 1u) density[2 * p + 1] += cb[ib];
+      if (bit(sb, ib, p)) density[2 * p + 1] += cb[ib];
 }
 
 void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody) {
@@ -520,11 +526,20 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
   auto stats = [&](auto lq) {
     constexpr int LQ = decltype(lq)::value;
     const unsigned ga = static_cast<unsigned>(na), gb = static_cast<unsigned>(nb);
-    k_rowstats<LQ><<<ga, kB, 0, st>>>(dW, nb, c.B.str.get(), L, rowsq.get(), ta.get());
     k_rowstats<1><<<gb, kB, 0, st>>>(wT.get(), na, c.A.str.get(), L, colsq.get(), nullptr);
-    SBDX_CHECK_LAUNCH();
-    k_pairdots<LQ, true, 1, 1><<<ga, kB, 0, st>>>(dW, nb, c.A.off_s.get(), c.A.ket_s.get(), c.B.str.get(), L, rs_a.get());
-    k_pairdots<LQ, true, 1, 1><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_s.get(), c.B.ket_s.get(), c.A.str.get(), L, rs_b.get());
+    // the spectator-weighted sums, one launch per string word (the plain sums of the second
+    // launch rewrite the same values)
+    with_words(c.W, [&](auto w) {
+      constexpr int SW = decltype(w)::value;
+      for (int word = 0; word < SW; ++word) {
+        k_rowstats<LQ, SW><<<ga, kB, 0, st>>>(dW, nb, c.B.str.get(), L, rowsq.get(), ta.get(), word);
+        SBDX_CHECK_LAUNCH();
+        k_pairdots<LQ, true, 1, 1, SW><<<ga, kB, 0, st>>>(dW, nb, c.A.off_s.get(), c.A.ket_s.get(), c.B.str.get(), L,
+                                                         rs_a.get(), word);
+        k_pairdots<LQ, true, 1, 1, SW><<<gb, kB, 0, st>>>(wT.get(), na, c.B.off_s.get(), c.B.ket_s.get(),
+                                                         c.A.str.get(), L, rs_b.get(), word);
+      }
+    });
     // the same-spin doubles: eight kets per wavefront, two columns per lane where the rows hold
     // whole 16-byte words
     auto doubles = [&](unsigned g, const double* w, int64_t ncol, const SpinTab& t, double* out) {
@@ -538,7 +553,7 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
   if (L <= 32) stats(std::integral_constant<int, 32>{});
   else if (L <= 48) stats(std::integral_constant<int, 48>{});
   else stats(std::integral_constant<int, 64>{});
-  const HostSpin A = host_spin(c.A), B = host_spin(c.B);
+  const HostSpin A = host_spin(c.A, c.W), B = host_spin(c.B, c.W);
   rdm_mixed(c, dW, A, B, G.get());
   SBDX_HIP(hipStreamSynchronize(st));
   auto hr = to_host(rowsq, na), hc = to_host(colsq, nb), ht = to_host(ta, na * L);
@@ -550,14 +565,16 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
   std::memset(twobody, 0, sizeof(double) * 4 * L4);
   Rdm r(L, onebody, twobody);
 
+  with_words(c.W, [&](auto w) {
+  constexpr int SW = decltype(w)::value;
   // diagonal (ZeroDiffCorrelation), grouped by spin pair
   for (int64_t ia = 0; ia < na; ++ia) {
-    const uint64_t s = A.str[ia];
+    const Str<SW> s = ld_str<SW>(A.str.data(), ia);
     for (int p = 0; p < L; ++p) {
-      if (!((s >> p) & 1u)) continue;
+      if (!test(s, p)) continue;
       r.g(0, p, p) += hr[ia];
       for (int q = 0; q < L; ++q) {
-        if (q == p || !((s >> q) & 1u)) continue;
+        if (q == p || !test(s, q)) continue;
         r.G(0, p, q, p, q) += hr[ia];
         r.G(0, p, q, q, p) -= hr[ia];
       }
@@ -569,12 +586,12 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
     }
   }
   for (int64_t ib = 0; ib < nb; ++ib) {
-    const uint64_t s = B.str[ib];
+    const Str<SW> s = ld_str<SW>(B.str.data(), ib);
     for (int p = 0; p < L; ++p) {
-      if (!((s >> p) & 1u)) continue;
+      if (!test(s, p)) continue;
       r.g(1, p, p) += hc[ib];
       for (int q = 0; q < L; ++q) {
-        if (q == p || !((s >> q) & 1u)) continue;
+        if (q == p || !test(s, q)) continue;
         r.G(3, p, q, p, q) += hc[ib];
         r.G(3, p, q, q, p) -= hc[ib];
       }
@@ -591,24 +608,20 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
     // expanded once per quadruple
     std::vector<double> dq(static_cast<size_t>(L4), 0.0);
     for (int64_t x = 0; x < nrow; ++x) {
-      const uint64_t sx = H.str[x];
+      const Str<SW> sx = ld_str<SW>(H.str.data(), x);
       for (int64_t k = H.off_s[x]; k < H.off_s[x + 1]; ++k) {
         const uint32_t e = H.exc_s[k];
-        const int q = e & 63, p = (e >> 6) & 63;  // q: ket-only (annihilated), p: bra-only
-        const double sg = (e >> 12) & 1u ? -1.0 : 1.0;
-        double wq[64];
+        const int q = exc_i(e), p = exc_a(e);  // q: ket-only (annihilated), p: bra-only
+        const double sg = exc_neg(e) ? -1.0 : 1.0;
+        double wq[128];
         for (int j = 0; j < L; ++j) wq[j] = sg * rs[k * (L + 1) + 1 + j];
         sp.add(p, q, sx, sg * rs[k * (L + 1)], wq);
       }
       for (int64_t k = H.off_d[x]; k < H.off_d[x + 1]; ++k) {
-        const uint64_t sy = H.str[H.ket_d[k]];
-        uint64_t cbits = sx & ~sy, dbits = sy & ~sx;
-        const int c0 = ctz64(cbits);
-        cbits &= cbits - 1;
-        const int c1 = ctz64(cbits);
-        const int d0 = ctz64(dbits);
-        dbits &= dbits - 1;
-        const int d1 = ctz64(dbits);
+        const Str<SW> sy = ld_str<SW>(H.str.data(), H.ket_d[k]);
+        const Str<SW> cbits = andn(sx, sy), dbits = andn(sy, sx);
+        const int c0 = first(cbits), c1 = first(drop_first(cbits));
+        const int d0 = first(dbits), d1 = first(drop_first(dbits));
         const int sg = sign_double(sx, c0, c1, d0, d1);
         dq[((static_cast<size_t>(c0) * L + c1) * L + d0) * L + d1] += sg * rd[k];
       }
@@ -622,6 +635,7 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
             if (w != 0.0) two_diff(r, 2 * c0 + spin, 2 * c1 + spin, 2 * d0 + spin, 2 * d1 + spin, w);
           }
   }
+  });
   // alpha single x beta single
   for (int a = 0; a < L; ++a)
     for (int i = 0; i < L; ++i)

@@ -228,13 +228,10 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
   // same-spin singles: value = stored part + sign * Coulomb table [pair, col]
   const int64_t s0 = rt.off_s[rb], s1 = rt.off_s[rb + 1];
   const T* __restrict__ cc = coul + col;
-  auto pair_of = [&](int64_t q) {
-    const uint32_t e = rt.exc_s[q];
-    return static_cast<int64_t>((e >> 6) & 63) * L + (e & 63);
-  };
+  auto pair_of = [&](int64_t q) { return static_cast<int64_t>(exc_pair(rt.exc_s[q], L)); };
   // the C columns' values of single q are distinct (their Coulomb terms differ)
   auto fma_single = [&](int64_t q, const VecK<T, C>& d, const VecK<T, KC>& xv) {
-    const bool neg = (rt.exc_s[q] >> 12) & 1u;
+    const bool neg = exc_neg(rt.exc_s[q]);
     const T base = val_s[q];
 #pragma unroll
     for (int c = 0; c < C; ++c) {
@@ -300,7 +297,9 @@ __host__ __device__ constexpr size_t mix_lds_bytes(int L, int max_s, int stages,
 // V row elements fetched per thread: kVPerThread * kMixThreads covers L^2 (2048: L <= 45,
 // 4096: L <= 64).  kStage alpha singles are staged per barrier (their V rows in LDS, their
 // gathers in flight together); the sums still take the alpha singles in ascending order, so
-// the result does not depend on kStage.
+// the result does not depend on kStage.  kVPerThread = 0 (more than 64 orbitals, where two V
+// rows of up to 16,384 values no longer fit the LDS): every entry reads its V value from the
+// row in global memory (L2-resident while the workgroup runs) instead of LDS.
 
 template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage, int kAcc, int kVW>
 __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
@@ -357,12 +356,13 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
   VecK<T, kVW> pre[kStage][kVV];
   bool pre_neg[kStage];
   auto fetch = [&](int kk) {
+    if constexpr (kVPerThread == 0) return;
 #pragma unroll
     for (int q = 0; q < kStage; ++q) {
       if (kk + q >= nk) break;
       const uint32_t e = __builtin_amdgcn_readfirstlane(s_exc[kk + q]);
-      const T* src = V + static_cast<int64_t>(((e >> 6) & 63) * L + (e & 63)) * L2;
-      pre_neg[q] = (e >> 12) & 1u;
+      const T* src = V + static_cast<int64_t>(exc_pair(e, L)) * L2;
+      pre_neg[q] = exc_neg(e);
 #pragma unroll
       for (int u = 0; u < kVV; ++u) {
         const int p = (tid + u * kMixThreads) * kVW;
@@ -371,6 +371,7 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
     }
   };
   auto commit = [&](T* dst, int kk) {
+    if constexpr (kVPerThread == 0) return;
 #pragma unroll
     for (int q = 0; q < kStage; ++q) {
       if (kk + q >= nk) break;
@@ -415,9 +416,19 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
 #pragma unroll
       for (int q = 0; q < kStage; ++q) {
         if (kk + q >= nk) break;
+        const T* vrow;
+        bool vneg = false;
+        if constexpr (kVPerThread == 0) {
+          const uint32_t e = __builtin_amdgcn_readfirstlane(s_exc[kk + q]);
+          vrow = V + static_cast<int64_t>(exc_pair(e, L)) * L2;
+          vneg = exc_neg(e);
+        } else {
+          vrow = cur + q * L2;
+        }
 #pragma unroll
         for (int j = 0; j < kEPT; ++j) {
-          const T w = cur[q * L2 + ((meta[j] >> 16) & 0xfff)];
+          T w = vrow[(meta[j] >> 16) & 0x3fff];
+          if constexpr (kVPerThread == 0) w = vneg ? -w : w;
 #pragma unroll
           for (int v = 0; v < K; ++v) acc[j][v].fma(w, xv[q][j].v[v]);
         }
@@ -516,7 +527,7 @@ __global__ __launch_bounds__(256) void k_add_transposed(const T* __restrict__ in
   }
 }
 
-template <typename T, int K>
+template <typename T, int K, int W>
 __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__ sa,
                                                     const uint64_t* __restrict__ sb, int64_t na,
                                                     int64_t nb, const T* __restrict__ x,
@@ -525,7 +536,7 @@ __global__ __launch_bounds__(256) void k_apply_sign(const uint64_t* __restrict__
   const int64_t ia = blockIdx.x / nt;
   const int64_t ib = (blockIdx.x % nt) * 256 + threadIdx.x;
   if (ib >= nb) return;
-  const bool neg = det_sign(sa[ia], sb[ib]) < 0;
+  const bool neg = det_sign(ld_str<W>(sa, ia), ld_str<W>(sb, ib)) < 0;
   const int64_t k = (ia * nb + ib) * K;
   VecK<T, K> a = ldk<T, K>(x + k);
 #pragma unroll
@@ -553,6 +564,7 @@ int builtin_variant(const sbdx_ctx& c, int prec, int K) {
     if (K == 1) return 51216;
     if (K == 2) return 1051208;
   }
+  if (c.L > 64) return K == 1 ? 25616 : K == 2 ? 25608 : 25604;  // V rows from global memory
   if (c.L * c.L > 2048) {
     if (K == 1) return 25616;
     if (K == 2) return 25608;
@@ -629,9 +641,9 @@ const MixTiles& mix_tiles(sbdx_ctx& c, int64_t cap) {
     for (int64_t b = start; b < ib; ++b)
       for (int64_t k = off[b]; k < off[b + 1]; ++k) {
         const uint32_t e = exc[k];
-        const uint32_t pb = ((e >> 6) & 63) * static_cast<uint32_t>(L) + (e & 63);
+        const uint32_t pb = exc_pair(e, L);
         const uint32_t slot = static_cast<uint32_t>(k - off[start]);
-        const uint32_t meta = slot | (pb << 16) | (((e >> 12) & 1u) << 31);
+        const uint32_t meta = slot | (pb << 16) | (exc_neg(e) ? 1u << 31 : 0u);
         buf.push_back(Ent{ket[k], meta});
       }
     std::stable_sort(buf.begin(), buf.end(), [](const Ent& u, const Ent& v) { return u.jb < v.jb; });
@@ -661,9 +673,11 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   if (mt.ntile == 0) return;
   const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
   using Wide = typename AccOf<T, kAcc>::type::Wide;
+  // more than 64 orbitals: V rows read from global memory, none staged in LDS
+  const bool global_v = c.L > 64;
   auto launch = [&](auto kernel, int threads, int ept) {
-    const size_t lds = mix_lds_bytes(c.L, c.A.max_s, stages, static_cast<int>(sizeof(T)), threads * ept,
-                                     static_cast<int>(sizeof(Wide)));
+    const size_t lds = mix_lds_bytes(c.L, c.A.max_s, global_v ? 0 : stages, static_cast<int>(sizeof(T)),
+                                     threads * ept, static_cast<int>(sizeof(Wide)));
     if (c.nb * K * static_cast<int64_t>(sizeof(T)) >= (int64_t(1) << 32))
       throw Error(SBDX_ERR_ARG, "beta string count too large for 32-bit ket-row offsets of the mixed kernel");
     if (lds > kMixMaxLds) throw Error(SBDX_ERR_ARG, "mixed-kernel variant needs more LDS than a CU has for this problem");
@@ -681,7 +695,10 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   const bool wide_v = (static_cast<int64_t>(c.L) * c.L * sizeof(T)) % 16 == 0;
   auto pick = [&](auto e, auto t, auto sg) {
     constexpr int E = decltype(e)::value, NT = decltype(t)::value, SG = decltype(sg)::value;
-    if (wide_v)
+    if (global_v) {
+      if constexpr (SG == 1 && NT == 256) launch(&k_sigma_mixed<T, E, 0, K, NT, 1, kAcc, 1>, NT, E);
+      else throw Error(SBDX_ERR_ARG, "above 64 orbitals the mixed kernel runs only its 256-thread one-stage variants");
+    } else if (wide_v)
       small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc, VW>, NT, E)
               : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc, VW>, NT, E);
     else
@@ -938,13 +955,17 @@ void sigma_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
 template <typename T>
 void sign_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
   const unsigned blocks = static_cast<unsigned>(c.na * cdiv(c.nb, 256));
-  switch (k) {
-    case 1: k_apply_sign<T, 1><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
-    case 2: k_apply_sign<T, 2><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
-    case 4: k_apply_sign<T, 4><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
-    case 8: k_apply_sign<T, 8><<<blocks, 256, 0, st>>>(c.A.str.get(), c.B.str.get(), c.na, c.nb, x, y); break;
-    default: throw Error(-1, "block width must be 1, 2, 4 or 8");
-  }
+  with_words(c.W, [&](auto w) {
+    constexpr int W = decltype(w)::value;
+    const uint64_t *sa = c.A.str.get(), *sb = c.B.str.get();
+    switch (k) {
+      case 1: k_apply_sign<T, 1, W><<<blocks, 256, 0, st>>>(sa, sb, c.na, c.nb, x, y); break;
+      case 2: k_apply_sign<T, 2, W><<<blocks, 256, 0, st>>>(sa, sb, c.na, c.nb, x, y); break;
+      case 4: k_apply_sign<T, 4, W><<<blocks, 256, 0, st>>>(sa, sb, c.na, c.nb, x, y); break;
+      case 8: k_apply_sign<T, 8, W><<<blocks, 256, 0, st>>>(sa, sb, c.na, c.nb, x, y); break;
+      default: throw Error(-1, "block width must be 1, 2, 4 or 8");
+    }
+  });
   SBDX_CHECK_LAUNCH();
 }
 
@@ -978,6 +999,7 @@ bool mix_variant_fits(const sbdx_ctx& c, int prec, int K, int var) {
   if (var < K) return mix_variant_fits(c, prec, var, fused_variant(c, prec, var));  // fused sub-blocks
   const int stages = var / 1000000 + 1, es = prec == 0 ? 8 : 4;
   const int threads = (var % 1000000) / 100, ept = var % 100;
+  if (c.L > 64) return stages == 1 && threads == 256 && mix_lds_bytes(c.L, c.A.max_s, 0, es, threads * ept, 8) <= kMixMaxLds;
   return mix_lds_bytes(c.L, c.A.max_s, stages, es, threads * ept, 8) <= kMixMaxLds;
 }
 

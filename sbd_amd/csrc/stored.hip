@@ -46,20 +46,17 @@ __global__ void k_csr_len(Lists A, Lists B, int64_t na, int64_t nb, int64_t* __r
 }
 
 // value of a single excitation entry q of one spin, with the spectator string's Coulomb term
-__device__ inline double single_value(const Lists& T, int64_t q, uint64_t spectator, const double* V, int L) {
+template <int W>
+__device__ inline double single_value(const Lists& T, int64_t q, const Str<W>& spectator, const double* V, int L) {
   const uint32_t e = T.exc_s[q];
   const int64_t L2 = static_cast<int64_t>(L) * L;
-  const double* vp = V + (static_cast<int64_t>((e >> 6) & 63) * L + (e & 63)) * L2;
+  const double* vp = V + static_cast<int64_t>(exc_pair(e, L)) * L2;
   double coul = 0.0;
-  uint64_t o = spectator;
-  while (o) {
-    const int j = ctz64(o);
-    o &= o - 1;
-    coul += vp[j * L + j];
-  }
-  return T.val_s[q] + ((e >> 12) & 1u ? -coul : coul);
+  for_each_bit(spectator, [&](int j) { coul += vp[j * L + j]; });
+  return T.val_s[q] + (exc_neg(e) ? -coul : coul);
 }
 
+template <int W>
 __global__ __launch_bounds__(64 * kWaves) void k_csr_fill(Lists A, Lists B, int64_t na, int64_t nb, int L,
                                                           const double* __restrict__ V,
                                                           const double* __restrict__ hii,
@@ -85,7 +82,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_csr_fill(Lists A, Lists B, int6
     }
     p += ad1 - ad0;
     const int64_t as0 = A.off_s[ia], as1 = A.off_s[ia + 1];
-    const uint64_t bstr = B.str[ib], astr = A.str[ia];
+    const Str<W> bstr = ld_str<W>(B.str, ib), astr = ld_str<W>(A.str, ia);
     for (int64_t q = as0 + lane; q < as1; q += 64) {
       col[p + q - as0] = static_cast<uint32_t>(static_cast<int64_t>(A.ket_s[q]) * nb + ib);
       val[p + q - as0] = single_value(A, q, bstr, V, L);
@@ -109,9 +106,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_csr_fill(Lists A, Lists B, int6
     for (int64_t t = lane; t < npair; t += 64) {
       const int64_t qa = as0 + t / ns, qb = bs0 + t % ns;
       const uint32_t ea = A.exc_s[qa], eb = B.exc_s[qb];
-      const double v = V[(static_cast<int64_t>((ea >> 6) & 63) * L + (ea & 63)) * L2 +
-                         static_cast<int64_t>((eb >> 6) & 63) * L + (eb & 63)];
-      const bool neg = (((ea >> 12) ^ (eb >> 12)) & 1u) != 0;
+      const double v = V[static_cast<int64_t>(exc_pair(ea, L)) * L2 + exc_pair(eb, L)];
+      const bool neg = exc_neg(ea) != exc_neg(eb);
       col[p + t] = static_cast<uint32_t>(static_cast<int64_t>(A.ket_s[qa]) * nb + B.ket_s[qb]);
       val[p + t] = neg ? -v : v;
     }
@@ -243,8 +239,10 @@ int64_t store_hamiltonian(sbdx_ctx& c) {
   c.h_val.alloc(nnz);
   c.h_val32.release();
   const unsigned g = static_cast<unsigned>(std::min<int64_t>(cdiv(n, kWaves), 65536));
-  k_csr_fill<<<g, 64 * kWaves, 0, st>>>(A, B, c.na, c.nb, c.L, c.eri4.get(), c.hii.get(), rowptr.get(), c.h_col.get(),
-                                        c.h_val.get());
+  with_words(c.W, [&](auto w) {
+    k_csr_fill<decltype(w)::value><<<g, 64 * kWaves, 0, st>>>(A, B, c.na, c.nb, c.L, c.eri4.get(), c.hii.get(),
+                                                              rowptr.get(), c.h_col.get(), c.h_val.get());
+  });
   SBDX_CHECK_LAUNCH();
   SBDX_HIP(hipStreamSynchronize(st));
   c.h_rowptr = std::move(rowptr);

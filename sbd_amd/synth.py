@@ -34,7 +34,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 __all__ = [
-    "Integrals", "Instance", "make_integrals", "full_space", "fermi_strings",
+    "Integrals", "Instance", "make_integrals", "full_space", "fermi_strings", "as_words",
+    "make_window", "embed",
     "pack_eri", "unpack_eri", "fcidump_entries", "make_instance", "WORKLOADS",
 ]
 
@@ -57,7 +58,7 @@ class Integrals:
 class Instance:
     name: str
     ints: Integrals
-    alpha: np.ndarray       # uint64 strings, sorted, unique
+    alpha: np.ndarray       # uint64 strings, sorted, unique; (n, 2) words above 64 orbitals
     beta: np.ndarray
     n_alpha_el: int
     n_beta_el: int
@@ -69,7 +70,7 @@ class Instance:
 
     @property
     def ndet(self) -> int:
-        return int(self.alpha.size) * int(self.beta.size)
+        return int(self.alpha.shape[0]) * int(self.beta.shape[0])
 
 
 def make_integrals(norb: int, rank: int, seed: int) -> Integrals:
@@ -97,7 +98,9 @@ def full_space(norb: int, nel: int) -> np.ndarray:
 
 def fermi_strings(norb: int, nel: int, count: int, eps: np.ndarray, temperature: float,
                   seed: int, batch: int | None = None) -> np.ndarray:
-    """Fermi-weighted strings, ``count`` unique ones (see module docstring)."""
+    """Fermi-weighted strings, ``count`` unique ones (see module docstring).  Up to 64 orbitals
+    a sorted uint64 array; above, a (count, 2) uint64 array of (low word, high word) rows
+    sorted by numeric value."""
     n_possible = 1
     for k in range(nel):
         n_possible = n_possible * (norb - k) // (k + 1)
@@ -109,7 +112,8 @@ def fermi_strings(norb: int, nel: int, count: int, eps: np.ndarray, temperature:
     x = np.clip((eps - mu) / temperature, -700.0, 700.0)
     logw = -np.log1p(np.exp(x))
     weights = np.exp(logw)
-    pw = np.where(np.arange(norb) < 64, np.uint64(1) << np.arange(norb, dtype=np.uint64), 0)
+    if norb <= 64:
+        pw = np.uint64(1) << np.arange(norb, dtype=np.uint64)
     batch = batch or max(4 * count, 4096)
     seen: dict[int, None] = {}
     while len(seen) < count:
@@ -117,13 +121,19 @@ def fermi_strings(norb: int, nel: int, count: int, eps: np.ndarray, temperature:
         g = -np.log(-np.log(np.clip(u, 1e-300, 1.0 - 1e-16)))
         keys = np.log(weights)[None, :] + g
         occ = np.argpartition(-keys, nel - 1, axis=1)[:, :nel]
-        masks = np.bitwise_or.reduce(pw[occ], axis=1)
-        for m in masks.tolist():
+        if norb <= 64:
+            masks = np.bitwise_or.reduce(pw[occ], axis=1).tolist()
+        else:
+            masks = [sum(1 << int(p) for p in row) for row in occ]
+        for m in masks:
             if m not in seen:
                 seen[m] = None
                 if len(seen) == count:
                     break
-    return np.array(sorted(seen.keys()), dtype=np.uint64)
+    keys = sorted(seen.keys())
+    if norb <= 64:
+        return np.array(keys, dtype=np.uint64)
+    return np.array([[m & 0xFFFFFFFFFFFFFFFF, m >> 64] for m in keys], dtype=np.uint64)
 
 
 def _pair(i, j):
@@ -206,6 +216,55 @@ def make_instance(name: str, strings: int | None = None, seed_ints: int = 0,
                     alpha=alpha, beta=beta, n_alpha_el=nel, n_beta_el=nel,
                     meta=dict(seed_ints=seed_ints, seed_alpha=seed_alpha,
                               seed_beta=seed_beta, rank=rank))
+
+
+def as_words(masks, norb: int) -> np.ndarray:
+    """Python-int string masks -> the engine's layout: a uint64 array up to 64 orbitals, else
+    (n, 2) rows of (low word, high word)."""
+    masks = [int(m) for m in masks]
+    if norb <= 64:
+        return np.array(masks, dtype=np.uint64)
+    return np.array([[m & 0xFFFFFFFFFFFFFFFF, m >> 64] for m in masks], dtype=np.uint64).reshape(-1, 2)
+
+
+def masks_of(strings: np.ndarray) -> list:
+    """The inverse of as_words."""
+    if strings.ndim == 1:
+        return [int(m) for m in strings.tolist()]
+    return [int(lo) | (int(hi) << 64) for lo, hi in strings.tolist()]
+
+
+def make_window(norb: int, lo: int, hi: int, n_alpha_el: int, n_beta_el: int, rank: int,
+                seed_ints: int = 0, name: str = "window") -> Instance:
+    """Every string of each spin with its electrons in orbitals [lo, hi) (the full space of the
+    window, sorted), with the recipe's integrals of all norb orbitals.  A window across orbital
+    64 puts the excitations of the strings across the two words of a 128-orbital string."""
+    ints = make_integrals(norb, rank, seed_ints)
+    strings = lambda nel: as_words([int(m) << lo for m in full_space(hi - lo, nel).tolist()], norb)
+    return Instance(name=name, ints=ints, alpha=strings(n_alpha_el), beta=strings(n_beta_el),
+                    n_alpha_el=n_alpha_el, n_beta_el=n_beta_el,
+                    meta=dict(seed_ints=seed_ints, rank=rank, window=[lo, hi]))
+
+
+def embed(inst: Instance, norb: int, shift: int, seed_ints: int = 99) -> Instance:
+    """The same problem with orbital p renamed p + shift among norb orbitals: the integrals of
+    the orbitals [shift, shift + L0) are inst's, all others come from the recipe (seed_ints) and
+    never enter, since no string occupies them.  Its H_ii, sigma, energies and RDMs (at the
+    renamed indices) are those of inst."""
+    L0 = inst.norb
+    if shift + L0 > norb:
+        raise ValueError("the renamed orbitals must fit in norb")
+    big = make_integrals(norb, 8, seed_ints)
+    blk = slice(shift, shift + L0)
+    h1 = big.h1.copy()
+    h1[blk, blk] = inst.ints.h1
+    eri = big.eri.copy()
+    eri[blk, blk, blk, blk] = inst.ints.eri
+    ints = Integrals(norb=norb, h1=h1, eri=eri, ecore=inst.ints.ecore)
+    move = lambda st: as_words([m << shift for m in masks_of(st)], norb)
+    return Instance(name=f"{inst.name}_at{shift}", ints=ints, alpha=move(inst.alpha),
+                    beta=move(inst.beta), n_alpha_el=inst.n_alpha_el, n_beta_el=inst.n_beta_el,
+                    meta=dict(inst.meta, embed_shift=shift, embed_norb=norb))
 
 
 def make_custom(norb: int, n_alpha_el: int, n_beta_el: int, rank: int,

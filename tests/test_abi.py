@@ -54,6 +54,29 @@ def test_create_rejects_bad_input_on_the_host(norb, alpha, beta, msg):
     assert e.value.code == -1 and msg in str(e.value)
 
 
+@pytest.mark.parametrize("norb,words,alpha,beta,msg", [
+    (129, 2, [[1, 0]], [[1, 0]], "norb"),
+    (65, 1, [[1]], [[1]], "norb"),
+    (70, 3, [[1, 0, 0]], [[1, 0, 0]], "words"),
+    (70, 2, [[1, 1 << 6]], [[1, 0]], "bits >= norb"),          # orbital 70 of 70
+    (70, 2, [[3, 0], [1, 3 << 4]], [[3, 0]], "electron count"),  # 2 and 1 + 2 electrons
+])
+def test_create_words_rejects_bad_input_on_the_host(norb, words, alpha, beta, msg):
+    """sbdx_create_words checks sizes, stray bits in every word and electron counts summed over
+    the words before it touches the GPU."""
+    a = np.array(alpha, np.uint64)
+    b = np.array(beta, np.uint64)
+    with pytest.raises(_lib.SbdxError) as e:
+        _lib.Engine(norb, a, b)
+    assert e.value.code == -1 and msg in str(e.value)
+
+
+def test_create_points_above_64_orbitals_to_create_words():
+    with pytest.raises(_lib.SbdxError) as e:
+        _lib.Engine(70, np.array([1], np.uint64), np.array([1], np.uint64))
+    assert "sbdx_create_words" in str(e.value)
+
+
 def test_null_arguments():
     lib = _lib.lib()
     assert lib.sbdx_create(4, 1, np.array([3], np.uint64), 1, np.array([3], np.uint64), None) == -1

@@ -2,7 +2,9 @@
 
 Lists bit-exact; H_ii and sigma to 1e-12 relative; Davidson energies to 1e-9 Ha; RDMs of the
 converged vector to 1e-6 (both solves stop at a 1e-8 residual); RDMs of a seeded vector to
-1e-12 (36 and 48 orbitals).
+1e-12 (36, 48 and 70 orbitals).  w70_3a3b: 70 orbitals, every 3-electron string of orbitals
+59..68 per spin, so the strings and their excitations cross from the first 64-bit word into
+the second (the reference keeps strings of any length in bit_length-bit words).
 """
 import json
 import os
@@ -16,7 +18,7 @@ from sbd_amd._lib import Engine
 pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["l6_2a2b", "l8_3a2b", "c2fam_120x110", "c2fam_300x280", "c3fam_200x220"]
+CASES = ["l6_2a2b", "l8_3a2b", "c2fam_120x110", "c2fam_300x280", "c3fam_200x220", "w70_3a3b"]
 
 
 def load(name):
@@ -25,9 +27,14 @@ def load(name):
 
 
 def build(meta):
-    inst = synth.make_custom(meta["norb"], meta["n_alpha_el"], meta["n_beta_el"], meta["rank"],
-                             meta["n_alpha"], meta["n_beta"], seed_ints=meta["seed_ints"],
-                             seed_alpha=meta["seed_alpha"], seed_beta=meta["seed_beta"])
+    if meta["recipe"] == "sbd_amd.synth.make_window":
+        lo, hi = meta["window"]
+        inst = synth.make_window(meta["norb"], lo, hi, meta["n_alpha_el"], meta["n_beta_el"], meta["rank"],
+                                 seed_ints=meta["seed_ints"])
+    else:
+        inst = synth.make_custom(meta["norb"], meta["n_alpha_el"], meta["n_beta_el"], meta["rank"],
+                                 meta["n_alpha"], meta["n_beta"], seed_ints=meta["seed_ints"],
+                                 seed_alpha=meta["seed_alpha"], seed_beta=meta["seed_beta"])
     return inst, Engine.from_instance(inst)
 
 
@@ -71,7 +78,7 @@ def test_golden_davidson_and_rdm(name):
     assert np.max(np.abs(eng.occupation(r["W"]) - z["density"])) < 1e-6
 
 
-@pytest.mark.parametrize("name", ["c2fam_300x280", "c3fam_200x220"])
+@pytest.mark.parametrize("name", ["c2fam_300x280", "c3fam_200x220", "w70_3a3b"])
 def test_golden_rdm_seeded_vector(name):
     """Correlation and OccupationDensity of one normalised seeded vector: every stored 2-RDM
     entry (the largest 150,000 or all), the whole of the 1-RDM and density, and whole-array
