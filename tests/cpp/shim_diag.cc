@@ -5,8 +5,9 @@
 // against the CPU oracle.
 //
 //   shim_diag <instance file> <out prefix> [tolerance]
-// Instance file: char[4] "SBDI", u32 1, i32 L, nel_a, nel_b, pad, i64 na, nb, nent,
-//                u64 alpha[na], u64 beta[nb], f64 val[nent], i32 idx[4 nent]
+// Instance file: char[4] "SBDI", u32 version, i32 L, nel_a, nel_b, words (version 1: one),
+//                i64 na, nb, nent, u64 alpha[na words], u64 beta[nb words], f64 val[nent],
+//                i32 idx[4 nent]; a string is `words` uint64 words, least significant first
 #include <cstdio>
 #include <fstream>
 #include <string>
@@ -30,11 +31,12 @@ int main(int argc, char** argv) {
   f.read(reinterpret_cast<char*>(&ver), 4);
   f.read(reinterpret_cast<char*>(hdr), 16);
   f.read(reinterpret_cast<char*>(n3), 24);
-  std::vector<uint64_t> a(n3[0]), b(n3[1]);
+  const int W = ver == 2 ? hdr[3] : 1;
+  std::vector<uint64_t> a(n3[0] * W), b(n3[1] * W);
   std::vector<double> val(n3[2]);
   std::vector<int32_t> idx(4 * n3[2]);
-  f.read(reinterpret_cast<char*>(a.data()), 8 * n3[0]);
-  f.read(reinterpret_cast<char*>(b.data()), 8 * n3[1]);
+  f.read(reinterpret_cast<char*>(a.data()), 8 * n3[0] * W);
+  f.read(reinterpret_cast<char*>(b.data()), 8 * n3[1] * W);
   f.read(reinterpret_cast<char*>(val.data()), 8 * n3[2]);
   f.read(reinterpret_cast<char*>(idx.data()), 16 * n3[2]);
   if (!f) return 3;
@@ -48,9 +50,9 @@ int main(int argc, char** argv) {
     fcidump.integrals.emplace_back(val[e], idx[4 * e], idx[4 * e + 1], idx[4 * e + 2], idx[4 * e + 3]);
   auto words = [&](const std::vector<uint64_t>& s) {
     std::vector<std::vector<size_t>> r;
-    for (uint64_t x : s) {
+    for (size_t k = 0; k < s.size() / W; ++k) {
       std::string str;
-      for (int p = L - 1; p >= 0; --p) str += ((x >> p) & 1u) ? '1' : '0';
+      for (int p = L - 1; p >= 0; --p) str += ((s[k * W + p / 64] >> (p % 64)) & 1u) ? '1' : '0';
       r.push_back(sbd::from_string(str, 20, L));
     }
     return r;

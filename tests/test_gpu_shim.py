@@ -110,3 +110,37 @@ def test_cpp_drop_in_stored_hamiltonian(tmp_path):
     ref = O.System.from_instance(inst).davidson(max_it=200, num_block=10, eps=1e-8)
     assert abs(res["energy"] - ref["energy"]) < 1e-9
     assert "Stored Hamiltonian elements" in r.stdout
+
+
+def test_cpp_drop_in_api_above_64_orbitals(tmp_path):
+    """The driver on the 70-orbital window problem of the compiled reference's fixture
+    (tests/golden/w70_3a3b.npz: every 3-electron string of orbitals 59..68 per spin, so the
+    header packs each bit_length-20 string into two 64-bit words): energy, H_ii and the RDMs of
+    the converged vector against the reference's own outputs."""
+    if not os.path.exists(BIN):
+        pytest.skip("tests/cpp/shim_diag not built (needs the image's MPI)")
+    from sbd_amd._lib import Engine
+    z = np.load(os.path.join(ROOT, "tests", "golden", "w70_3a3b.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    lo, hi = meta["window"]
+    inst = synth.make_window(meta["norb"], lo, hi, 3, 3, meta["rank"], seed_ints=meta["seed_ints"])
+    vals, idx = inst.ints.fcidump_entries()
+    path = str(tmp_path / "inst.bin")
+    O.write_instance(path, inst.norb, 3, 3, inst.alpha, inst.beta, vals, idx)
+    out = str(tmp_path / "shim")
+    print("[shim w70]", flush=True)
+    r = subprocess.run([BIN, path, out, "1e-8"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    assert res["ndet"] == inst.ndet
+    assert abs(res["energy"] - float(z["energy"][0])) < 1e-9
+    hii = np.fromfile(out + ".hii")
+    assert np.max(np.abs(hii[z["hii_idx"]] - z["hii_val"])) < 1e-12 * np.abs(hii).max()
+    x = np.cos(0.37 * np.arange(inst.ndet))
+    want = 1.0 + Engine.from_instance(inst).sigma(x)
+    assert np.max(np.abs(np.fromfile(out + ".multy") - want)) < 1e-14 * np.abs(want).max()
+    assert np.max(np.abs(np.fromfile(out + ".one") - z["one"])) < 1e-6
+    two = np.fromfile(out + ".two")
+    assert two.size == 4 * 70 ** 4
+    assert np.max(np.abs(two[z["two_idx"]] - z["two_val"])) < 1e-6
+    assert np.max(np.abs(np.fromfile(out + ".density") - z["density"])) < 1e-6
