@@ -123,7 +123,7 @@ def nnz_parts(eng):
     """Exact off-diagonal element counts of one sigma (same-spin alpha, same-spin beta, mixed)."""
     sa, da = eng.excitation_counts(0)
     sb, db = eng.excitation_counts(1)
-    na, nb = eng.alpha.size, eng.beta.size
+    na, nb = eng.alpha.shape[0], eng.beta.shape[0]
     return dict(aa=nb * (sa + da), bb=na * (sb + db), ab=sa * sb)
 
 
@@ -133,7 +133,7 @@ def compulsory_bytes(eng, w):
     FP32; H_ii, the list values and the integrals are held in the same precision)."""
     sa, da = eng.excitation_counts(0)
     sb, db = eng.excitation_counts(1)
-    na, nb = eng.alpha.size, eng.beta.size
+    na, nb = eng.alpha.shape[0], eng.beta.shape[0]
     N = na * nb
     V = w * eng.norb ** 4  # dense (pq|rs)
 
@@ -184,7 +184,7 @@ def cpu_sigma_time(inst, threads):
     so = O.System.from_instance(inst)
     sa = int(so.conn_a[0][-1]); da = int(so.conn_a[2][-1])
     sb = int(so.conn_b[0][-1]); db = int(so.conn_b[2][-1])
-    nnz = inst.beta.size * (sa + da) + inst.alpha.size * (sb + db) + sa * sb
+    nnz = inst.beta.shape[0] * (sa + da) + inst.alpha.shape[0] * (sb + db) + sa * sb
     if O.have_reference():
         try:
             line, _ = O.run_reference(inst, "timemult", threads=threads, extra=["--reps", "1"])
@@ -390,6 +390,11 @@ def main():
                     help="autotune the mixed pass of each --block width before timing it")
     ap.add_argument("--block", type=int, nargs="*", default=[],
                     help="also time the fused sigma of blocks of these widths (1, 2, 4, 8)")
+    ap.add_argument("--variant", type=int, default=0,
+                    help="mixed-kernel variant of the timed one-vector build (sbdx_set_mix_variant; 0: default)")
+    ap.add_argument("--rename", type=int, nargs=2, default=None, metavar=("NORB", "SHIFT"),
+                    help="the same problem with orbital p renamed p + SHIFT among NORB orbitals "
+                         "(NORB > 64: two-word strings; same H, same sigma)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -407,13 +412,20 @@ def main():
 
     t = time.perf_counter()
     inst = synth.make_instance(args.workload, strings=args.strings)
+    renamed = ""
+    if args.rename:
+        L0 = inst.norb
+        inst = synth.embed(inst, args.rename[0], args.rename[1])
+        renamed = f", orbitals renamed p -> p + {args.rename[1]} among {inst.norb} (from {L0})"
     t_gen = time.perf_counter() - t
-    log(f"[bench] instance {inst.name}: L={inst.norb} {inst.alpha.size}x{inst.beta.size} "
+    log(f"[bench] instance {inst.name}: L={inst.norb} {inst.alpha.shape[0]}x{inst.beta.shape[0]} "
         f"= {inst.ndet} dets (generated in {t_gen:.1f}s)")
     torch.cuda.synchronize()
     t = time.perf_counter()
     eng = Engine.from_instance(inst)
     eng.set_fp32_accumulation(1 if args.fp32_acc == "twosum" else 0)
+    if args.variant:
+        eng.set_mix_variant(SBDX_FP64 if args.prec == "fp64" else SBDX_FP32, 1, args.variant)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t
     parts = nnz_parts(eng)
@@ -460,7 +472,7 @@ def main():
     gath = gathered_bytes(parts, N, w)
     dom = max(phase_ms, key=phase_ms.get)
     dom_s = phase_ms[dom] * 1e-3
-    pmc = counters(args.workload, args.strings, args.prec).get(dom, {})
+    pmc = {} if args.rename else counters(args.workload, args.strings, args.prec).get(dom, {})
     # live ceiling of HBM: a STREAM-style copy of 4 GiB
     hbm_probe = bandwidth_probe(0, 4 << 30, 5)
     log(f"[bench] probe: HBM copy {hbm_probe:.0f} GB/s")
@@ -509,12 +521,13 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"{inst.name}: L={inst.norb}, {inst.n_alpha_el}a{inst.n_beta_el}b, "
-                         f"rank-{inst.meta['rank']} integrals, {inst.alpha.size}x{inst.beta.size}"
-                         f" Fermi-sampled strings, {N} determinants"),
+                         f"rank-{inst.meta['rank']} integrals, {inst.alpha.shape[0]}x{inst.beta.shape[0]}"
+                         f" Fermi-sampled strings, {N} determinants{renamed}"),
             "ndet": N,
             "offdiag_elements": nnz,
             "precision": args.prec,
             "fp32_accumulation": args.fp32_acc,
+            **({"mix_variant": args.variant} if args.variant else {}),
         },
         "roofline": roofline,
         "sigma_roofline": {

@@ -297,8 +297,9 @@ __host__ __device__ constexpr size_t mix_lds_bytes(int L, int max_s, int stages,
 // V row elements fetched per thread: kVPerThread * kMixThreads covers L^2 (2048: L <= 45,
 // 4096: L <= 64).  kStage alpha singles are staged per barrier (their V rows in LDS, their
 // gathers in flight together); the sums still take the alpha singles in ascending order, so
-// the result does not depend on kStage.  kVPerThread = 0 (more than 64 orbitals, where two V
-// rows of up to 16,384 values no longer fit the LDS): every entry reads its V value from the
+// the result does not depend on kStage.  More than 64 orbitals: 16,384 / kMixThreads covers
+// L^2 <= 16,384 (the 1024-thread one-stage variants, where two such rows fit the LDS: FP64 up
+// to ~99 orbitals, FP32 up to 128), or kVPerThread = 0: every entry reads its V value from the
 // row in global memory (L2-resident while the workgroup runs) instead of LDS.
 
 template <typename T, int kEPT, int kVPerThread, int K, int kMixThreads, int kStage, int kAcc, int kVW>
@@ -366,7 +367,11 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
 #pragma unroll
       for (int u = 0; u < kVV; ++u) {
         const int p = (tid + u * kMixThreads) * kVW;
-        pre[q][u] = ldk<T, kVW>(src + (p < L2 ? p : 0));
+        if constexpr (kVPerThread * kMixThreads == 16384) {  // rows of 4,097..16,384 values
+          if (p < L2) pre[q][u] = ldk<T, kVW>(src + p);       // (commit skips the rest)
+        } else {
+          pre[q][u] = ldk<T, kVW>(src + (p < L2 ? p : 0));
+        }
       }
     }
   };
@@ -564,7 +569,14 @@ int builtin_variant(const sbdx_ctx& c, int prec, int K) {
     if (K == 1) return 51216;
     if (K == 2) return 1051208;
   }
-  if (c.L > 64) return K == 1 ? 25616 : K == 2 ? 25608 : 25604;  // V rows from global memory
+  if (c.L > 64) {
+    // V rows staged in LDS by the 1024-thread variants where two rows fit, else read from
+    // global memory by the 256-thread ones
+    const int es = prec == 0 ? 8 : 4;
+    if (K == 1) return mix_lds_bytes(c.L, c.A.max_s, 1, es, 1024 * 8, 8) <= kMixMaxLds ? 102408 : 25616;
+    if (K == 2) return mix_lds_bytes(c.L, c.A.max_s, 1, es, 1024 * 4, 8) <= kMixMaxLds ? 102404 : 25608;
+    return 25604;
+  }
   if (c.L * c.L > 2048) {
     if (K == 1) return 25616;
     if (K == 2) return 25608;
@@ -587,6 +599,7 @@ int mix_variant(const sbdx_ctx& c, int prec, int K) {
   const int tuned = tuned_variant(c, prec, K);
   if (tuned) return tuned;
   if (K >= 4 && c.L * c.L <= 2048) return kSplit2;
+  if (K >= 4 && c.L > 64 && builtin_variant(c, prec, 2) == 102404) return kSplit2;  // LDS-staged pairs
   return fused_variant(c, prec, K);
 }
 
@@ -673,8 +686,9 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   if (mt.ntile == 0) return;
   const int64_t blocks = 8 * cdiv(c.na, 8) * mt.ntile;
   using Wide = typename AccOf<T, kAcc>::type::Wide;
-  // more than 64 orbitals: V rows read from global memory, none staged in LDS
-  const bool global_v = c.L > 64;
+  // more than 64 orbitals: the 1024-thread one-stage variants stage V rows of up to 16,384
+  // values in LDS (where two fit), the 256-thread ones read them from global memory
+  const bool global_v = c.L > 64 && threads == 256;
   auto launch = [&](auto kernel, int threads, int ept) {
     const size_t lds = mix_lds_bytes(c.L, c.A.max_s, global_v ? 0 : stages, static_cast<int>(sizeof(T)),
                                      threads * ept, static_cast<int>(sizeof(Wide)));
@@ -695,9 +709,15 @@ void launch_mixed(sbdx_ctx& c, const T* x, T* y, const T* V, hipStream_t st) {
   const bool wide_v = (static_cast<int64_t>(c.L) * c.L * sizeof(T)) % 16 == 0;
   auto pick = [&](auto e, auto t, auto sg) {
     constexpr int E = decltype(e)::value, NT = decltype(t)::value, SG = decltype(sg)::value;
-    if (global_v) {
-      if constexpr (SG == 1 && NT == 256) launch(&k_sigma_mixed<T, E, 0, K, NT, 1, kAcc, 1>, NT, E);
-      else throw Error(SBDX_ERR_ARG, "above 64 orbitals the mixed kernel runs only its 256-thread one-stage variants");
+    if (c.L > 64) {
+      if constexpr (SG == 1 && NT == 256) {
+        launch(&k_sigma_mixed<T, E, 0, K, NT, 1, kAcc, 1>, NT, E);
+      } else if constexpr (SG == 1 && NT == 1024) {
+        if (wide_v) launch(&k_sigma_mixed<T, E, 16384 / NT, K, NT, 1, kAcc, VW>, NT, E);
+        else launch(&k_sigma_mixed<T, E, 16384 / NT, K, NT, 1, kAcc, 1>, NT, E);
+      } else {
+        throw Error(SBDX_ERR_ARG, "above 64 orbitals the mixed kernel runs its one-stage 256- and 1024-thread variants only");
+      }
     } else if (wide_v)
       small_v ? launch(&k_sigma_mixed<T, E, 2048 / NT, K, NT, SG, kAcc, VW>, NT, E)
               : launch(&k_sigma_mixed<T, E, 4096 / NT, K, NT, SG, kAcc, VW>, NT, E);
@@ -999,7 +1019,11 @@ bool mix_variant_fits(const sbdx_ctx& c, int prec, int K, int var) {
   if (var < K) return mix_variant_fits(c, prec, var, fused_variant(c, prec, var));  // fused sub-blocks
   const int stages = var / 1000000 + 1, es = prec == 0 ? 8 : 4;
   const int threads = (var % 1000000) / 100, ept = var % 100;
-  if (c.L > 64) return stages == 1 && threads == 256 && mix_lds_bytes(c.L, c.A.max_s, 0, es, threads * ept, 8) <= kMixMaxLds;
+  if (c.L > 64) {  // one stage; 256 threads: V from global memory, 1024: V rows in LDS
+    if (stages != 1 || (threads != 256 && threads != 1024)) return false;
+    if (threads == 1024 && ept * K > 8) return false;  // would spill: 128 registers at 1024 threads
+    return mix_lds_bytes(c.L, c.A.max_s, threads == 256 ? 0 : 1, es, threads * ept, 8) <= kMixMaxLds;
+  }
   return mix_lds_bytes(c.L, c.A.max_s, stages, es, threads * ept, 8) <= kMixMaxLds;
 }
 

@@ -90,3 +90,25 @@ def test_new_integrals_drop_the_stored_h(stored):
     assert np.max(np.abs(eng.sigma(x) - ref)) <= 1e-12 * np.max(np.abs(ref))
     assert eng.store_hamiltonian(False) == 0
     eng.set_integrals(inst.ints.ecore, I1.ravel(), inst.ints.packed_eri())
+
+
+def test_stored_above_64_orbitals():
+    """Two-word strings: the stored H of the 70-orbital window problem (every 3-electron string
+    of orbitals 59..68 per spin) gives the on-the-fly build's sigma (itself pinned against the
+    compiled reference, tests/test_gpu_golden.py w70_3a3b), and the Davidson energy of the
+    reference's fixture."""
+    print("[stored w70]", flush=True)
+    import json
+    import os
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "w70_3a3b.npz"), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    lo, hi = meta["window"]
+    inst = synth.make_window(meta["norb"], lo, hi, 3, 3, meta["rank"], seed_ints=meta["seed_ints"])
+    fly, sto = Engine.from_instance(inst), Engine.from_instance(inst)
+    sto.store_hamiltonian(True)
+    for seed in (0, 1):
+        x = np.random.default_rng(seed).standard_normal(inst.ndet)
+        ref = fly.sigma(x)
+        assert np.max(np.abs(sto.sigma(x) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    r = sto.davidson(max_iteration=100, num_block=10, eps=1e-8)
+    assert r["converged"] and abs(r["energy"] - float(z["energy"][0])) < 1e-9

@@ -82,6 +82,22 @@ def test_block_sigma_identical(pair, prec, k):
     assert np.array_equal(block_sigma(e0, prec, X), block_sigma(e1, prec, X))
 
 
+@pytest.mark.parametrize("k,variant", [(1, 25616), (2, 25608), (4, 25604)])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_integral_rows_from_global_memory(pair, prec, k, variant):
+    """Above 64 orbitals the default variants stage each integral row in LDS; the 256-thread
+    variants read it from global memory instead.  Both give the one-word build's sigma."""
+    pytest.importorskip("torch")
+    e0, e1 = pair
+    print(f"[wide global-V prec={prec} K={k} variant={variant}]", flush=True)
+    X = np.random.default_rng(20 + k).standard_normal((e0.ndet, k))
+    e1.set_mix_variant(prec, k, variant)
+    try:
+        assert np.array_equal(block_sigma(e0, prec, X), block_sigma(e1, prec, X))
+    finally:
+        e1.set_mix_variant(prec, k, 0)
+
+
 def test_davidson_identical(pair):
     e0, e1 = pair
     print("[wide davidson]", flush=True)
