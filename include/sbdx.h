@@ -64,9 +64,10 @@ int sbdx_create(int norb, int64_t n_alpha, const uint64_t *alpha, int64_t n_beta
 
 /* sbdx_create for strings of `words` (1 or 2) uint64 words each, least significant word first
  * (orbital p = bit p % 64 of word p / 64), norb <= 64 words <= 128; bits at or above norb must
- * be clear.  Above 64 orbitals the mixed pass reads its integral rows from global memory
- * (they no longer fit the LDS twice), runs its 256-thread one-stage variants only, and the
- * learned preconditioner is not available (its inputs hold 64 orbitals per spin). */
+ * be clear.  Above 64 orbitals the mixed pass runs its one-stage variants: 1024 threads stage
+ * each integral row of L^2 values in LDS where two rows fit (FP64 up to ~99 orbitals, FP32 up
+ * to 128), 256 threads read it from global memory; the learned preconditioner is not available
+ * (its inputs hold 64 orbitals per spin). */
 int sbdx_create_words(int norb, int words, int64_t n_alpha, const uint64_t *alpha, int64_t n_beta,
                       const uint64_t *beta, sbdx_ctx **out);
 
