@@ -228,7 +228,10 @@ __global__ __launch_bounds__(kRowTile) void k_sigma_rows(
   // same-spin singles: value = stored part + sign * Coulomb table [pair, col]
   const int64_t s0 = rt.off_s[rb], s1 = rt.off_s[rb + 1];
   const T* __restrict__ cc = coul + col;
-  auto pair_of = [&](int64_t q) { return static_cast<int64_t>(exc_pair(rt.exc_s[q], L)); };
+  auto pair_of = [&](int64_t q) {  // a L + i, in 64 bits (the form that keeps the loads interleaved)
+    const uint32_t e = rt.exc_s[q];
+    return static_cast<int64_t>(exc_a(e)) * L + exc_i(e);
+  };
   // the C columns' values of single q are distinct (their Coulomb terms differ)
   auto fma_single = [&](int64_t q, const VecK<T, C>& d, const VecK<T, KC>& xv) {
     const bool neg = exc_neg(rt.exc_s[q]);
@@ -367,11 +370,7 @@ __global__ __launch_bounds__(kMixThreads) void k_sigma_mixed(
 #pragma unroll
       for (int u = 0; u < kVV; ++u) {
         const int p = (tid + u * kMixThreads) * kVW;
-        if constexpr (kVPerThread * kMixThreads == 16384) {  // rows of 4,097..16,384 values
-          if (p < L2) pre[q][u] = ldk<T, kVW>(src + p);       // (commit skips the rest)
-        } else {
-          pre[q][u] = ldk<T, kVW>(src + (p < L2 ? p : 0));
-        }
+        if (p < L2) pre[q][u] = ldk<T, kVW>(src + p);  // (commit skips the rest)
       }
     }
   };
