@@ -529,8 +529,8 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
     k_rowstats<1><<<gb, kB, 0, st>>>(wT.get(), na, c.A.str.get(), L, colsq.get(), nullptr);
     // the spectator-weighted sums, one launch per string word (the plain sums of the second
     // launch rewrite the same values)
-    with_words(c.W, [&](auto w) {
-      constexpr int SW = decltype(w)::value;
+    with_words(c.W, [&](auto words) {
+      constexpr int SW = decltype(words)::value;
       for (int word = 0; word < SW; ++word) {
         k_rowstats<LQ, SW><<<ga, kB, 0, st>>>(dW, nb, c.B.str.get(), L, rowsq.get(), ta.get(), word);
         SBDX_CHECK_LAUNCH();
@@ -565,76 +565,76 @@ void rdm_blocked(sbdx_ctx& c, const double* dW, double* onebody, double* twobody
   std::memset(twobody, 0, sizeof(double) * 4 * L4);
   Rdm r(L, onebody, twobody);
 
-  with_words(c.W, [&](auto w) {
-  constexpr int SW = decltype(w)::value;
-  // diagonal (ZeroDiffCorrelation), grouped by spin pair
-  for (int64_t ia = 0; ia < na; ++ia) {
-    const Str<SW> s = ld_str<SW>(A.str.data(), ia);
-    for (int p = 0; p < L; ++p) {
-      if (!test(s, p)) continue;
-      r.g(0, p, p) += hr[ia];
-      for (int q = 0; q < L; ++q) {
-        if (q == p || !test(s, q)) continue;
-        r.G(0, p, q, p, q) += hr[ia];
-        r.G(0, p, q, q, p) -= hr[ia];
-      }
-      for (int q = 0; q < L; ++q) {
-        const double v = ht[ia * L + q];
-        r.G(0 + 2 * 1, p, q, p, q) += v;
-        r.G(1 + 2 * 0, q, p, q, p) += v;
-      }
-    }
-  }
-  for (int64_t ib = 0; ib < nb; ++ib) {
-    const Str<SW> s = ld_str<SW>(B.str.data(), ib);
-    for (int p = 0; p < L; ++p) {
-      if (!test(s, p)) continue;
-      r.g(1, p, p) += hc[ib];
-      for (int q = 0; q < L; ++q) {
-        if (q == p || !test(s, q)) continue;
-        r.G(3, p, q, p, q) += hc[ib];
-        r.G(3, p, q, q, p) -= hc[ib];
+  with_words(c.W, [&](auto words) {
+    constexpr int SW = decltype(words)::value;
+    // diagonal (ZeroDiffCorrelation), grouped by spin pair
+    for (int64_t ia = 0; ia < na; ++ia) {
+      const Str<SW> s = ld_str<SW>(A.str.data(), ia);
+      for (int p = 0; p < L; ++p) {
+        if (!test(s, p)) continue;
+        r.g(0, p, p) += hr[ia];
+        for (int q = 0; q < L; ++q) {
+          if (q == p || !test(s, q)) continue;
+          r.G(0, p, q, p, q) += hr[ia];
+          r.G(0, p, q, q, p) -= hr[ia];
+        }
+        for (int q = 0; q < L; ++q) {
+          const double v = ht[ia * L + q];
+          r.G(0 + 2 * 1, p, q, p, q) += v;
+          r.G(1 + 2 * 0, q, p, q, p) += v;
+        }
       }
     }
-  }
-  // same-spin singles and doubles
-  for (int spin = 0; spin < 2; ++spin) {
-    const HostSpin& H = spin == 0 ? A : B;
-    const std::vector<double>& rs = spin == 0 ? hsa : hsb;
-    const std::vector<double>& rd = spin == 0 ? hda : hdb;
-    const int64_t nrow = spin == 0 ? na : nb;
-    SinglesByPair sp(L);
-    // doubles summed per orbital quadruple (bra-only c0 < c1, ket-only d0 < d1) first, then
-    // expanded once per quadruple
-    std::vector<double> dq(static_cast<size_t>(L4), 0.0);
-    for (int64_t x = 0; x < nrow; ++x) {
-      const Str<SW> sx = ld_str<SW>(H.str.data(), x);
-      for (int64_t k = H.off_s[x]; k < H.off_s[x + 1]; ++k) {
-        const uint32_t e = H.exc_s[k];
-        const int q = exc_i(e), p = exc_a(e);  // q: ket-only (annihilated), p: bra-only
-        const double sg = exc_neg(e) ? -1.0 : 1.0;
-        double wq[128];
-        for (int j = 0; j < L; ++j) wq[j] = sg * rs[k * (L + 1) + 1 + j];
-        sp.add(p, q, sx, sg * rs[k * (L + 1)], wq);
-      }
-      for (int64_t k = H.off_d[x]; k < H.off_d[x + 1]; ++k) {
-        const Str<SW> sy = ld_str<SW>(H.str.data(), H.ket_d[k]);
-        const Str<SW> cbits = andn(sx, sy), dbits = andn(sy, sx);
-        const int c0 = first(cbits), c1 = first(drop_first(cbits));
-        const int d0 = first(dbits), d1 = first(drop_first(dbits));
-        const int sg = sign_double(sx, c0, c1, d0, d1);
-        dq[((static_cast<size_t>(c0) * L + c1) * L + d0) * L + d1] += sg * rd[k];
+    for (int64_t ib = 0; ib < nb; ++ib) {
+      const Str<SW> s = ld_str<SW>(B.str.data(), ib);
+      for (int p = 0; p < L; ++p) {
+        if (!test(s, p)) continue;
+        r.g(1, p, p) += hc[ib];
+        for (int q = 0; q < L; ++q) {
+          if (q == p || !test(s, q)) continue;
+          r.G(3, p, q, p, q) += hc[ib];
+          r.G(3, p, q, q, p) -= hc[ib];
+        }
       }
     }
-    sp.expand(r, spin);
-    for (int c0 = 0; c0 < L; ++c0)
-      for (int c1 = c0 + 1; c1 < L; ++c1)
-        for (int d0 = 0; d0 < L; ++d0)
-          for (int d1 = d0 + 1; d1 < L; ++d1) {
-            const double w = dq[((static_cast<size_t>(c0) * L + c1) * L + d0) * L + d1];
-            if (w != 0.0) two_diff(r, 2 * c0 + spin, 2 * c1 + spin, 2 * d0 + spin, 2 * d1 + spin, w);
-          }
-  }
+    // same-spin singles and doubles
+    for (int spin = 0; spin < 2; ++spin) {
+      const HostSpin& H = spin == 0 ? A : B;
+      const std::vector<double>& rs = spin == 0 ? hsa : hsb;
+      const std::vector<double>& rd = spin == 0 ? hda : hdb;
+      const int64_t nrow = spin == 0 ? na : nb;
+      SinglesByPair sp(L);
+      // doubles summed per orbital quadruple (bra-only c0 < c1, ket-only d0 < d1) first, then
+      // expanded once per quadruple
+      std::vector<double> dq(static_cast<size_t>(L4), 0.0);
+      for (int64_t x = 0; x < nrow; ++x) {
+        const Str<SW> sx = ld_str<SW>(H.str.data(), x);
+        for (int64_t k = H.off_s[x]; k < H.off_s[x + 1]; ++k) {
+          const uint32_t e = H.exc_s[k];
+          const int q = exc_i(e), p = exc_a(e);  // q: ket-only (annihilated), p: bra-only
+          const double sg = exc_neg(e) ? -1.0 : 1.0;
+          double wq[128];
+          for (int j = 0; j < L; ++j) wq[j] = sg * rs[k * (L + 1) + 1 + j];
+          sp.add(p, q, sx, sg * rs[k * (L + 1)], wq);
+        }
+        for (int64_t k = H.off_d[x]; k < H.off_d[x + 1]; ++k) {
+          const Str<SW> sy = ld_str<SW>(H.str.data(), H.ket_d[k]);
+          const Str<SW> cbits = andn(sx, sy), dbits = andn(sy, sx);
+          const int c0 = first(cbits), c1 = first(drop_first(cbits));
+          const int d0 = first(dbits), d1 = first(drop_first(dbits));
+          const int sg = sign_double(sx, c0, c1, d0, d1);
+          dq[((static_cast<size_t>(c0) * L + c1) * L + d0) * L + d1] += sg * rd[k];
+        }
+      }
+      sp.expand(r, spin);
+      for (int c0 = 0; c0 < L; ++c0)
+        for (int c1 = c0 + 1; c1 < L; ++c1)
+          for (int d0 = 0; d0 < L; ++d0)
+            for (int d1 = d0 + 1; d1 < L; ++d1) {
+              const double w = dq[((static_cast<size_t>(c0) * L + c1) * L + d0) * L + d1];
+              if (w != 0.0) two_diff(r, 2 * c0 + spin, 2 * c1 + spin, 2 * d0 + spin, 2 * d1 + spin, w);
+            }
+    }
   });
   // alpha single x beta single
   for (int a = 0; a < L; ++a)

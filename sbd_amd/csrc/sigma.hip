@@ -974,8 +974,8 @@ void sigma_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
 template <typename T>
 void sign_dispatch(sbdx_ctx& c, int k, const T* x, T* y, hipStream_t st) {
   const unsigned blocks = static_cast<unsigned>(c.na * cdiv(c.nb, 256));
-  with_words(c.W, [&](auto w) {
-    constexpr int W = decltype(w)::value;
+  with_words(c.W, [&](auto words) {
+    constexpr int W = decltype(words)::value;
     const uint64_t *sa = c.A.str.get(), *sb = c.B.str.get();
     switch (k) {
       case 1: k_apply_sign<T, 1, W><<<blocks, 256, 0, st>>>(sa, sb, c.na, c.nb, x, y); break;

@@ -239,8 +239,8 @@ int64_t store_hamiltonian(sbdx_ctx& c) {
   c.h_val.alloc(nnz);
   c.h_val32.release();
   const unsigned g = static_cast<unsigned>(std::min<int64_t>(cdiv(n, kWaves), 65536));
-  with_words(c.W, [&](auto w) {
-    k_csr_fill<decltype(w)::value><<<g, 64 * kWaves, 0, st>>>(A, B, c.na, c.nb, c.L, c.eri4.get(), c.hii.get(),
+  with_words(c.W, [&](auto words) {
+    k_csr_fill<decltype(words)::value><<<g, 64 * kWaves, 0, st>>>(A, B, c.na, c.nb, c.L, c.eri4.get(), c.hii.get(),
                                                               rowptr.get(), c.h_col.get(), c.h_val.get());
   });
   SBDX_CHECK_LAUNCH();

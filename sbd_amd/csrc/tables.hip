@@ -171,8 +171,8 @@ void build_spin(SpinTab& t, const uint64_t* host_strings, int64_t n, int L, int 
   DevBuf<int64_t> ns(n), nd(n);
   const int waves_per_block = 4;
   const dim3 grid(static_cast<unsigned>(cdiv(n, waves_per_block)));
-  with_words(W, [&](auto w) {
-    constexpr int WW = decltype(w)::value;
+  with_words(W, [&](auto words) {
+    constexpr int WW = decltype(words)::value;
     Str<WW> mask;
     for (int k = 0; k < WW; ++k) mask.w[k] = mask_range(0, std::min(64, std::max(0, L - 64 * k)));
     k_count_exc<WW><<<grid, waves_per_block * kWave, 0, st>>>(t.str.get(), n, mask, ns.get(), nd.get());
@@ -197,8 +197,8 @@ void build_spin(SpinTab& t, const uint64_t* host_strings, int64_t n, int L, int 
   t.ket_d.alloc(t.nnz_d);
   SBDX_HIP(hipMemcpyAsync(t.off_s.get(), os.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
   SBDX_HIP(hipMemcpyAsync(t.off_d.get(), od.data(), (n + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
-  with_words(W, [&](auto w) {
-    constexpr int WW = decltype(w)::value;
+  with_words(W, [&](auto words) {
+    constexpr int WW = decltype(words)::value;
     Str<WW> mask;
     for (int k = 0; k < WW; ++k) mask.w[k] = mask_range(0, std::min(64, std::max(0, L - 64 * k)));
     k_fill_exc<WW><<<grid, waves_per_block * kWave, 0, st>>>(t.str.get(), n, mask, t.off_s.get(), t.ket_s.get(),
@@ -214,8 +214,8 @@ void spin_values(SpinTab& t, int L, int W, const double* h1, const double* eri4,
   t.val_d.alloc(t.nnz_d);
   t.e1.alloc(t.n);
   const int bs = 256;
-  with_words(W, [&](auto w) {
-    constexpr int WW = decltype(w)::value;
+  with_words(W, [&](auto words) {
+    constexpr int WW = decltype(words)::value;
     if (t.nnz_s)
       k_single_values<WW><<<static_cast<unsigned>(cdiv(t.nnz_s, bs)), bs, 0, st>>>(
           t.str.get(), t.n, t.off_s.get(), t.ket_s.get(), t.nnz_s, L, h1, eri4, t.exc_s.get(), t.val_s.get());
@@ -294,8 +294,8 @@ void build_integral_tables(sbdx_ctx& c) {
   c.coul_b.alloc(L2 * c.na);
   c.coul_a32.release();
   c.coul_b32.release();
-  with_words(c.W, [&](auto w) {
-    constexpr int WW = decltype(w)::value;
+  with_words(c.W, [&](auto words) {
+    constexpr int WW = decltype(words)::value;
     k_hdiag<WW><<<static_cast<unsigned>(c.na * ntile), bs, 0, st>>>(
         c.A.str.get(), c.B.str.get(), c.na, c.nb, L, c.I0, c.A.e1.get(), c.B.e1.get(), c.eri4.get(),
         c.hii.get(), nullptr);
