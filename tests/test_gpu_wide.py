@@ -131,3 +131,28 @@ def test_rdm_and_occupation_at_renamed_indices():
     assert np.max(np.abs(d1[2 * s:2 * (s + L0)] - d0)) < 1e-13
     d1[2 * s:2 * (s + L0)] = 0.0
     assert not d1.any()
+
+
+def test_two_word_input_at_64_orbitals_or_fewer():
+    """sbdx_create_words with two words per string but at most 64 orbitals (high words zero)
+    builds the one-word context: lists, H_ii and sigma equal sbdx_create's."""
+    print("[wide two-word input, 10 orbitals]", flush=True)
+    inst = synth.make_custom(10, 3, 3, 8, 60, 50, seed_ints=3)
+    e0 = Engine.from_instance(inst)
+    wide = lambda s: np.stack([s, np.zeros_like(s)], axis=1)
+    e1 = Engine(inst.norb, wide(inst.alpha), wide(inst.beta), *ref_integrals(inst))
+    for spin in (0, 1):
+        for a, b in zip(e0.excitations(spin), e1.excitations(spin)):
+            assert np.array_equal(a, b)
+    assert np.array_equal(e0.hdiag(), e1.hdiag())
+    x = np.random.default_rng(4).standard_normal(e0.ndet)
+    assert np.array_equal(e0.sigma(x), e1.sigma(x))
+
+
+def ref_integrals(inst):
+    """(I0, I1, I2) of an instance in the reference's storage, as Engine.from_instance passes them."""
+    L = inst.norb
+    I1 = np.zeros((2 * L, 2 * L))
+    I1[0::2, 0::2] = inst.ints.h1
+    I1[1::2, 1::2] = inst.ints.h1
+    return inst.ints.ecore, I1.ravel(), inst.ints.packed_eri()
