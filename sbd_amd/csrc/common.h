@@ -152,6 +152,11 @@ struct sbdx_ctx {
   int mix_variant_tuned[3][4] = {}; // autotuned mixed-kernel variant [FP64|FP32|FP32 two-sum][K slot]
   int fp32_acc = SBDX_ACC_FP64;     // accumulation of FP32 sigma builds (sbdx_set_fp32_accumulation)
   bool stored = false;             // H stored as CSR (stored.hip): sigma builds use it
+  // determinant 0's block of H, for the block solver's start vectors: per spin, 1 for the
+  // strings in the connected component of string 0 (strings joined by a single or double
+  // excitation); state 0 = not built, 1 = both spins connected (no masks), 2 = masks built
+  int block0_state = 0;
+  sbdx::DevBuf<uint8_t> block0_a, block0_b;
   // set while sbdx_sigma_device_noalloc runs: the sigma path then takes its transposed pair
   // from noalloc_xT/yT and treats any allocation it would make as an error
   bool no_alloc = false;

@@ -406,8 +406,11 @@ __global__ __launch_bounds__(kB) void k_copy_col(double* __restrict__ dst, int d
 }
 
 // the kq smallest H_ii over i >= 1 per workgroup (kq <= 8): values and indices, ascending
+// the kq lowest H_ii of each workgroup's share, determinant 0 excluded; with masks (okA, okB)
+// only determinants whose alpha and beta strings are both marked
 __global__ __launch_bounds__(kB) void k_lowest(const double* __restrict__ hii, int64_t n, int kq,
-                                               double* __restrict__ val, int64_t* __restrict__ idx) {
+                                               const uint8_t* __restrict__ okA, const uint8_t* __restrict__ okB,
+                                               int64_t nb, double* __restrict__ val, int64_t* __restrict__ idx) {
   __shared__ double sv[kB * 8];
   __shared__ int64_t si[kB * 8];
   double v[8];
@@ -418,6 +421,7 @@ __global__ __launch_bounds__(kB) void k_lowest(const double* __restrict__ hii, i
   }
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x + 1; i < n;
        i += static_cast<int64_t>(gridDim.x) * kB) {
+    if (okA && !(okA[i / nb] && okB[i % nb])) continue;
     const double h = hii[i];
     if (h < v[kq - 1]) {
       int p = kq - 1;
@@ -461,6 +465,54 @@ __global__ __launch_bounds__(kB) void k_lowest(const double* __restrict__ hii, i
       idx[blockIdx.x * kq + a] = bi[a];
     }
   }
+}
+
+// Union-find over one spin's strings joined by their single and double excitations: 1 for the
+// strings in string 0's component.  Returns whether that component holds every string.
+bool component_of_string0(const SpinTab& t, std::vector<uint8_t>& mark) {
+  const int64_t n = t.n;
+  std::vector<int64_t> off_s(n + 1), off_d(n + 1);
+  std::vector<uint32_t> ket_s(std::max<int64_t>(t.nnz_s, 1)), ket_d(std::max<int64_t>(t.nnz_d, 1));
+  SBDX_HIP(hipMemcpy(off_s.data(), t.off_s.get(), (n + 1) * 8, hipMemcpyDeviceToHost));
+  SBDX_HIP(hipMemcpy(off_d.data(), t.off_d.get(), (n + 1) * 8, hipMemcpyDeviceToHost));
+  if (t.nnz_s) SBDX_HIP(hipMemcpy(ket_s.data(), t.ket_s.get(), t.nnz_s * 4, hipMemcpyDeviceToHost));
+  if (t.nnz_d) SBDX_HIP(hipMemcpy(ket_d.data(), t.ket_d.get(), t.nnz_d * 4, hipMemcpyDeviceToHost));
+  std::vector<int64_t> parent(n);
+  for (int64_t x = 0; x < n; ++x) parent[x] = x;
+  auto find = [&](int64_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+  };
+  for (int64_t x = 0; x < n; ++x) {
+    for (int64_t k = off_s[x]; k < off_s[x + 1]; ++k) parent[find(ket_s[k])] = find(x);
+    for (int64_t k = off_d[x]; k < off_d[x + 1]; ++k) parent[find(ket_d[k])] = find(x);
+  }
+  const int64_t r0 = find(0);
+  mark.assign(n, 0);
+  bool all = true;
+  for (int64_t x = 0; x < n; ++x) {
+    mark[x] = find(x) == r0;
+    all = all && mark[x];
+  }
+  return all;
+}
+
+// Determinant 0's block of H (built once per context): H couples no determinant inside it to
+// one outside, so a Davidson run started from determinant 0 alone -- the reference's, init 0 --
+// stays in it.  State 1: every string connected (the whole basis is one block).
+void build_block0(sbdx_ctx& c) {
+  if (c.block0_state) return;
+  std::vector<uint8_t> ma, mb;
+  const bool all_a = component_of_string0(c.A, ma), all_b = component_of_string0(c.B, mb);
+  if (all_a && all_b) {
+    c.block0_state = 1;
+    return;
+  }
+  c.block0_a.alloc(c.na);
+  c.block0_b.alloc(c.nb);
+  SBDX_HIP(hipMemcpy(c.block0_a.get(), ma.data(), c.na, hipMemcpyHostToDevice));
+  SBDX_HIP(hipMemcpy(c.block0_b.get(), mb.data(), c.nb, hipMemcpyHostToDevice));
+  c.block0_state = 2;
 }
 
 __global__ void k_set_entries(double* __restrict__ X, const int64_t* __restrict__ pos, int m) {
@@ -775,22 +827,33 @@ struct BlockSolver {
   void initial_block(const double* dW0) {
     SBDX_HIP(hipMemsetAsync(X.get(), 0, n * K * 8, st));
     if (K > 1) {
-      // columns 1..K-1: unit vectors on the K-1 lowest H_ii other than determinant 0
+      // columns 1..K-1: unit vectors on the K-1 lowest H_ii other than determinant 0; from the
+      // HF start (init 0), inside determinant 0's block of H where the basis splits into
+      // blocks (else a unit vector in another block can lead root 1 to that block's lowest
+      // eigenvalue, not the one the reference's Davidson finds from determinant 0), unless
+      // that block holds fewer than K determinants
       const int kq = K - 1;
       const unsigned g = nblk(n);
       DevBuf<double> val(g * kq);
       DevBuf<int64_t> idx(g * kq);
-      k_lowest<<<g, kB, 0, st>>>(c.hii.get(), n, kq, val.get(), idx.get());
-      SBDX_CHECK_LAUNCH();
-      std::vector<double> hv(g * kq);
-      std::vector<int64_t> hi(g * kq);
-      SBDX_HIP(hipMemcpyAsync(hv.data(), val.get(), hv.size() * 8, hipMemcpyDeviceToHost, st));
-      SBDX_HIP(hipMemcpyAsync(hi.data(), idx.get(), hi.size() * 8, hipMemcpyDeviceToHost, st));
-      SBDX_HIP(hipStreamSynchronize(st));
+      auto lowest = [&](const uint8_t* okA, const uint8_t* okB) {
+        k_lowest<<<g, kB, 0, st>>>(c.hii.get(), n, kq, okA, okB, c.nb, val.get(), idx.get());
+        SBDX_CHECK_LAUNCH();
+        std::vector<double> hv(g * kq);
+        std::vector<int64_t> hi(g * kq);
+        SBDX_HIP(hipMemcpyAsync(hv.data(), val.get(), hv.size() * 8, hipMemcpyDeviceToHost, st));
+        SBDX_HIP(hipMemcpyAsync(hi.data(), idx.get(), hi.size() * 8, hipMemcpyDeviceToHost, st));
+        SBDX_HIP(hipStreamSynchronize(st));
+        std::vector<std::pair<double, int64_t>> cand;
+        for (size_t q = 0; q < hv.size(); ++q)
+          if (hi[q] >= 0) cand.emplace_back(hv[q], hi[q]);
+        std::sort(cand.begin(), cand.end());
+        return cand;
+      };
+      if (o.init == 0) build_block0(c);
       std::vector<std::pair<double, int64_t>> cand;
-      for (size_t q = 0; q < hv.size(); ++q)
-        if (hi[q] >= 0) cand.emplace_back(hv[q], hi[q]);
-      std::sort(cand.begin(), cand.end());
+      if (o.init == 0 && c.block0_state == 2) cand = lowest(c.block0_a.get(), c.block0_b.get());
+      if (static_cast<int>(cand.size()) < kq) cand = lowest(nullptr, nullptr);
       if (static_cast<int>(cand.size()) < kq) throw Error(SBDX_ERR_ARG, "fewer determinants than the block width");
       std::vector<int64_t> pos(kq);
       for (int v = 1; v < K; ++v) pos[v - 1] = cand[v - 1].second * K + v;
