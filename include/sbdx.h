@@ -145,7 +145,11 @@ typedef struct {
   int method;             /* 0: the reference's algorithm (one vector, restart to the Ritz
                              vector every num_block steps); 1: block thick-restart Davidson */
   int block;              /* method 1: vectors per block (1, 2, 4, 8; rounded up to a power of
-                             two >= nroots); 0 = nroots */
+                             two >= nroots); 0 = nroots.  The first block: the start vector
+                             and unit vectors on the block - 1 lowest H_ii; with init 0 these
+                             are taken inside determinant 0's block of H (the determinants
+                             whose strings are linked to determinant 0's by chains of single
+                             and double excitations), where the reference's Davidson stays */
   int stop_at_tol;        /* 1: stop once energy_ref is reached (time-to-solution runs) */
   int precond_warmup;     /* preconditioner 1: restart cycles run with the diagonal
                              preconditioner before the learned one takes over (default 1:
