@@ -39,7 +39,17 @@ class Conn(C.Structure):
 
 class Basis(C.Structure):
     _fields_ = [("L", C.c_int), ("na", C.c_int64), ("nb", C.c_int64), ("a", C.c_void_p),
-                ("b", C.c_void_p), ("ca", Conn), ("cb", Conn)]
+                ("b", C.c_void_p), ("ca", Conn), ("cb", Conn), ("W", C.c_int)]
+
+
+DW = 4  # uint64 words of an interleaved determinant (SBDO_DW)
+
+
+def words_of(strings):
+    """(uint64 array of the strings, words per string): (n,) arrays hold one word per string,
+    (n, W) arrays W words, least significant first."""
+    s = np.ascontiguousarray(strings, dtype=np.uint64)
+    return s, (1 if s.ndim == 1 else s.shape[1])
 
 
 def build():
@@ -57,15 +67,15 @@ def lib():
         L.sbdo_setup_integrals.restype = C.c_int
         L.sbdo_value.argtypes = [C.POINTER(Ints), C.c_int, C.c_int, C.c_int, C.c_int]
         L.sbdo_value.restype = C.c_double
-        L.sbdo_det_from_ab.argtypes = [C.c_uint64, C.c_uint64, C.c_int, u64p]
+        L.sbdo_det_from_ab.argtypes = [u64p, u64p, C.c_int, u64p]
         L.sbdo_parity.argtypes = [u64p, C.c_int, C.c_int]
         L.sbdo_parity.restype = C.c_int
         L.sbdo_zero_excite.argtypes = [u64p, C.POINTER(Ints)]
         L.sbdo_zero_excite.restype = C.c_double
         L.sbdo_hij.argtypes = [u64p, u64p, C.POINTER(Ints)]
         L.sbdo_hij.restype = C.c_double
-        L.sbdo_count_excitations.argtypes = [u64p, C.c_int64, C.c_int, i64p, i64p]
-        L.sbdo_fill_excitations.argtypes = [u64p, C.c_int64, C.c_int, i64p, i64p, i64p, i64p]
+        L.sbdo_count_excitations.argtypes = [u64p, C.c_int64, C.c_int, C.c_int, i64p, i64p]
+        L.sbdo_fill_excitations.argtypes = [u64p, C.c_int64, C.c_int, C.c_int, i64p, i64p, i64p, i64p]
         L.sbdo_hdiag.argtypes = [C.POINTER(Basis), C.POINTER(Ints), f64p]
         L.sbdo_mult.argtypes = [C.POINTER(Basis), C.POINTER(Ints), f64p, f64p, f64p]
         L.sbdo_inner.argtypes = [f64p, f64p, C.c_int64]
@@ -99,18 +109,19 @@ def setup_integrals(norb, vals, idx):
 
 
 def excitations(strings, norb):
-    s = np.ascontiguousarray(strings, dtype=np.uint64)
-    n = s.size
+    s, W = words_of(strings)
+    n = s.shape[0]
+    s = s.reshape(-1)
     ns = np.zeros(n, np.int64)
     nd = np.zeros(n, np.int64)
-    lib().sbdo_count_excitations(s, n, norb, ns, nd)
+    lib().sbdo_count_excitations(s, n, W, norb, ns, nd)
     off_s = np.zeros(n + 1, np.int64)
     off_d = np.zeros(n + 1, np.int64)
     np.cumsum(ns, out=off_s[1:])
     np.cumsum(nd, out=off_d[1:])
     idx_s = np.zeros(max(int(off_s[-1]), 1), np.int64)
     idx_d = np.zeros(max(int(off_d[-1]), 1), np.int64)
-    lib().sbdo_fill_excitations(s, n, norb, off_s, idx_s, off_d, idx_d)
+    lib().sbdo_fill_excitations(s, n, W, norb, off_s, idx_s, off_d, idx_d)
     return off_s, idx_s[: off_s[-1]], off_d, idx_d[: off_d[-1]]
 
 
@@ -119,8 +130,10 @@ class System:
 
     def __init__(self, norb, alpha, beta, vals, idx):
         self.L = int(norb)
-        self.alpha = np.ascontiguousarray(alpha, dtype=np.uint64)
-        self.beta = np.ascontiguousarray(beta, dtype=np.uint64)
+        self.alpha, self.W = words_of(alpha)
+        self.beta, wb = words_of(beta)
+        if wb != self.W:
+            raise ValueError("alpha and beta strings must have the same word count")
         self.I0, self.I1, self.I2, self.Dm, self.Xm = setup_integrals(self.L, vals, idx)
         self.conn_a = excitations(self.alpha, self.L)
         self.conn_b = (self.conn_a if self.beta is self.alpha or np.array_equal(self.alpha, self.beta)
@@ -132,8 +145,8 @@ class System:
         cb = Conn(*[x.ctypes.data if x.size else np.zeros(1, np.int64).ctypes.data
                     for x in self.conn_b])
         self._keep = [np.zeros(1, np.int64) for _ in range(8)]
-        self._basis = Basis(self.L, self.alpha.size, self.beta.size, self.alpha.ctypes.data,
-                            self.beta.ctypes.data, ca, cb)
+        self._basis = Basis(self.L, self.alpha.shape[0], self.beta.shape[0], self.alpha.ctypes.data,
+                            self.beta.ctypes.data, ca, cb, self.W)
 
     @classmethod
     def from_instance(cls, inst):
@@ -142,14 +155,16 @@ class System:
 
     @property
     def ndet(self):
-        return self.alpha.size * self.beta.size
+        return self.alpha.shape[0] * self.beta.shape[0]
 
     def value(self, i, j, k, l):
         return lib().sbdo_value(C.byref(self._ints), i, j, k, l)
 
     def det(self, a, b):
-        d = np.zeros(2, np.uint64)
-        lib().sbdo_det_from_ab(int(a), int(b), self.L, d)
+        """Interleaved determinant (DW words) of an alpha and a beta string (an int, or W words)."""
+        d = np.zeros(DW, np.uint64)
+        nw = (self.L + 63) // 64
+        lib().sbdo_det_from_ab(string_words(a, nw), string_words(b, nw), self.L, d)
         return d
 
     def hij(self, bra, ket):
@@ -194,6 +209,16 @@ class System:
         d = np.zeros(2 * self.L)
         lib().sbdo_occupation(C.byref(self._basis), np.ascontiguousarray(W, np.float64), d)
         return d
+
+
+def string_words(x, nw):
+    """One string as nw uint64 words, least significant first (from an int of any width, or an
+    array of words, padded with zero words)."""
+    if isinstance(x, (int, np.integer)):
+        x = int(x)
+        return np.array([(x >> (64 * k)) & 0xFFFFFFFFFFFFFFFF for k in range(nw)], np.uint64)
+    w = np.ascontiguousarray(x, np.uint64).reshape(-1)
+    return np.concatenate([w, np.zeros(max(0, nw - w.size), np.uint64)])
 
 
 def set_threads(n):

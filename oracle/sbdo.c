@@ -70,21 +70,21 @@ int sbdo_setup_integrals(int L, int64_t nent, const double *val, const int32_t *
 
 /* ---------------------------------------------------------------- bits */
 
-static inline int getbit(const uint64_t d[2], int x) { return (int)((d[x >> 6] >> (x & 63)) & 1u); }
+static inline int getbit(const uint64_t d[SBDO_DW], int x) { return (int)((d[x >> 6] >> (x & 63)) & 1u); }
 
 /* determinants.h:98-119: alpha orbital p -> bit 2p, beta orbital p -> bit 2p+1 */
-void sbdo_det_from_ab(uint64_t a, uint64_t b, int L, uint64_t det[2]) {
-  det[0] = det[1] = 0;
+void sbdo_det_from_ab(const uint64_t *a, const uint64_t *b, int L, uint64_t det[SBDO_DW]) {
+  for (int w = 0; w < SBDO_DW; ++w) det[w] = 0;
   for (int p = 0; p < L; ++p) {
-    if ((a >> p) & 1u) det[(2 * p) >> 6] |= (uint64_t)1 << ((2 * p) & 63);
-    if ((b >> p) & 1u) det[(2 * p + 1) >> 6] |= (uint64_t)1 << ((2 * p + 1) & 63);
+    if ((a[p >> 6] >> (p & 63)) & 1u) det[(2 * p) >> 6] |= (uint64_t)1 << ((2 * p) & 63);
+    if ((b[p >> 6] >> (p & 63)) & 1u) det[(2 * p + 1) >> 6] |= (uint64_t)1 << ((2 * p + 1) & 63);
   }
 }
 
 /* popcount of bits in [start, end) */
-static inline int popcount_range(const uint64_t d[2], int start, int end) {
+static inline int popcount_range(const uint64_t d[SBDO_DW], int start, int end) {
   int n = 0;
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < SBDO_DW; ++w) {
     int lo = w * 64, hi = lo + 64;
     int s = start > lo ? start : lo, e = end < hi ? end : hi;
     if (s >= e) continue;
@@ -96,7 +96,7 @@ static inline int popcount_range(const uint64_t d[2], int start, int end) {
 }
 
 /* determinants.h:205-256: (-1)^{#set bits in [start,end)}, flipped again if bit start is set */
-int sbdo_parity(const uint64_t det[2], int start, int end) {
+int sbdo_parity(const uint64_t det[SBDO_DW], int start, int end) {
   int s = (popcount_range(det, start, end) & 1) ? -1 : 1;
   if (getbit(det, start)) s = -s;
   return s;
@@ -105,11 +105,11 @@ int sbdo_parity(const uint64_t det[2], int start, int end) {
 /* ---------------------------------------------------------------- Slater-Condon */
 
 /* determinants.h:374-398 */
-double sbdo_zero_excite(const uint64_t det[2], const sbdo_ints *in) {
+double sbdo_zero_excite(const uint64_t det[SBDO_DW], const sbdo_ints *in) {
   const int n2 = 2 * in->L, L = in->L;
-  int closed[128];
+  int closed[256];
   int nc = 0;
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < SBDO_DW; ++w) {
     uint64_t bits = det[w];
     while (bits) {
       int pos = __builtin_ctzll(bits);
@@ -132,11 +132,11 @@ double sbdo_zero_excite(const uint64_t det[2], const sbdo_ints *in) {
 }
 
 /* determinants.h:400-422: ket = det, i occupied in ket only, a occupied in bra only */
-static double one_excite(const uint64_t det[2], int i, int a, const sbdo_ints *in) {
+static double one_excite(const uint64_t det[SBDO_DW], int i, int a, const sbdo_ints *in) {
   const int n2 = 2 * in->L;
   double sgn = (double)sbdo_parity(det, i < a ? i : a, i < a ? a : i);
   double energy = in->I1[a * n2 + i];
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < SBDO_DW; ++w) {
     uint64_t bits = det[w];
     while (bits) {
       int pos = __builtin_ctzll(bits);
@@ -150,7 +150,7 @@ static double one_excite(const uint64_t det[2], int i, int a, const sbdo_ints *i
 }
 
 /* determinants.h:424-442 */
-static double two_excite(const uint64_t det[2], int i, int j, int a, int b, const sbdo_ints *in) {
+static double two_excite(const uint64_t det[SBDO_DW], int i, int j, int a, int b, const sbdo_ints *in) {
   double sgn = 1.0;
   int I = i < j ? i : j, J = i < j ? j : i;
   int A = a < b ? a : b, B = a < b ? b : a;
@@ -161,10 +161,10 @@ static double two_excite(const uint64_t det[2], int i, int j, int a, int b, cons
 }
 
 /* determinants.h:505-564: Hij(bra, ket) */
-double sbdo_hij(const uint64_t bra[2], const uint64_t ket[2], const sbdo_ints *in) {
+double sbdo_hij(const uint64_t bra[SBDO_DW], const uint64_t ket[SBDO_DW], const sbdo_ints *in) {
   int c[4], d[4];
   int nc = 0, nd = 0;
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < SBDO_DW; ++w) {
     uint64_t dc = bra[w] & ~ket[w], dd = ket[w] & ~bra[w];
     while (dc) {
       int pos = __builtin_ctzll(dc);
@@ -187,19 +187,25 @@ double sbdo_hij(const uint64_t bra[2], const uint64_t ket[2], const sbdo_ints *i
 
 /* ---------------------------------------------------------------- connectivity */
 
-static inline int string_difference(uint64_t x, uint64_t y, int L) {
-  uint64_t m = (L >= 64) ? ~(uint64_t)0 : (((uint64_t)1 << L) - 1);
-  return __builtin_popcountll((x ^ y) & m);
+/* orbitals (below L) in which two strings of W words differ */
+static inline int string_difference(const uint64_t *x, const uint64_t *y, int W, int L) {
+  int d = 0;
+  for (int w = 0; w < W; ++w) {
+    const int bits = L - 64 * w;
+    const uint64_t m = bits >= 64 ? ~(uint64_t)0 : (bits <= 0 ? 0 : (((uint64_t)1 << bits) - 1));
+    d += __builtin_popcountll((x[w] ^ y[w]) & m);
+  }
+  return d;
 }
 
 /* helper.h:132-182 */
-void sbdo_count_excitations(const uint64_t *s, int64_t n, int L, int64_t *nsingle,
+void sbdo_count_excitations(const uint64_t *s, int64_t n, int W, int L, int64_t *nsingle,
                             int64_t *ndouble) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t i = 0; i < n; ++i) {
     int64_t ns = 0, nd = 0;
     for (int64_t j = 0; j < n; ++j) {
-      int d = string_difference(s[i], s[j], L);
+      int d = string_difference(s + i * W, s + j * W, W, L);
       if (d == 2) ns++;
       else if (d == 4) nd++;
     }
@@ -208,13 +214,13 @@ void sbdo_count_excitations(const uint64_t *s, int64_t n, int L, int64_t *nsingl
   }
 }
 
-void sbdo_fill_excitations(const uint64_t *s, int64_t n, int L, const int64_t *off_s,
+void sbdo_fill_excitations(const uint64_t *s, int64_t n, int W, int L, const int64_t *off_s,
                            int64_t *idx_s, const int64_t *off_d, int64_t *idx_d) {
 #pragma omp parallel for schedule(dynamic, 64)
   for (int64_t i = 0; i < n; ++i) {
     int64_t ps = off_s[i], pd = off_d[i];
     for (int64_t j = 0; j < n; ++j) {
-      int d = string_difference(s[i], s[j], L);
+      int d = string_difference(s + i * W, s + j * W, W, L);
       if (d == 2) idx_s[ps++] = j;
       else if (d == 4) idx_d[pd++] = j;
     }
@@ -228,9 +234,9 @@ void sbdo_hdiag(const sbdo_basis *bs, const sbdo_ints *in, double *hii) {
   const int64_t na = bs->na, nb = bs->nb;
 #pragma omp parallel for schedule(static)
   for (int64_t ia = 0; ia < na; ++ia) {
-    uint64_t det[2];
+    uint64_t det[SBDO_DW];
     for (int64_t ib = 0; ib < nb; ++ib) {
-      sbdo_det_from_ab(bs->a[ia], bs->b[ib], bs->L, det);
+      sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + ib * bs->W, bs->L, det);
       hii[ia * nb + ib] = sbdo_zero_excite(det, in);
     }
   }
@@ -245,32 +251,32 @@ void sbdo_mult(const sbdo_basis *bs, const sbdo_ints *in, const double *hii,
   const sbdo_conn *A = &bs->ca, *Bc = &bs->cb;
 #pragma omp parallel for schedule(dynamic, 1)
   for (int64_t ia = 0; ia < na; ++ia) {
-    uint64_t deti[2], detj[2];
+    uint64_t deti[SBDO_DW], detj[SBDO_DW];
     for (int64_t ib = 0; ib < nb; ++ib) {
       const int64_t bra = ia * nb + ib;
       double acc = Wb[bra];
       acc += hii[bra] * Wk[bra];
-      sbdo_det_from_ab(bs->a[ia], bs->b[ib], L, deti);
+      sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + ib * bs->W, L, deti);
       /* type 2: alpha singles then alpha doubles (mult.h:498-532) */
       for (int64_t k = A->off_s[ia]; k < A->off_s[ia + 1]; ++k) {
         int64_t ja = A->idx_s[k];
-        sbdo_det_from_ab(bs->a[ja], bs->b[ib], L, detj);
+        sbdo_det_from_ab(bs->a + ja * bs->W, bs->b + ib * bs->W, L, detj);
         acc += sbdo_hij(deti, detj, in) * Wk[ja * nb + ib];
       }
       for (int64_t k = A->off_d[ia]; k < A->off_d[ia + 1]; ++k) {
         int64_t ja = A->idx_d[k];
-        sbdo_det_from_ab(bs->a[ja], bs->b[ib], L, detj);
+        sbdo_det_from_ab(bs->a + ja * bs->W, bs->b + ib * bs->W, L, detj);
         acc += sbdo_hij(deti, detj, in) * Wk[ja * nb + ib];
       }
       /* type 1: beta singles then beta doubles (mult.h:534-566) */
       for (int64_t k = Bc->off_s[ib]; k < Bc->off_s[ib + 1]; ++k) {
         int64_t jb = Bc->idx_s[k];
-        sbdo_det_from_ab(bs->a[ia], bs->b[jb], L, detj);
+        sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + jb * bs->W, L, detj);
         acc += sbdo_hij(deti, detj, in) * Wk[ia * nb + jb];
       }
       for (int64_t k = Bc->off_d[ib]; k < Bc->off_d[ib + 1]; ++k) {
         int64_t jb = Bc->idx_d[k];
-        sbdo_det_from_ab(bs->a[ia], bs->b[jb], L, detj);
+        sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + jb * bs->W, L, detj);
         acc += sbdo_hij(deti, detj, in) * Wk[ia * nb + jb];
       }
       /* type 0: alpha single x beta single (mult.h:569-596) */
@@ -278,7 +284,7 @@ void sbdo_mult(const sbdo_basis *bs, const sbdo_ints *in, const double *hii,
         int64_t ja = A->idx_s[k];
         for (int64_t m = Bc->off_s[ib]; m < Bc->off_s[ib + 1]; ++m) {
           int64_t jb = Bc->idx_s[m];
-          sbdo_det_from_ab(bs->a[ja], bs->b[jb], L, detj);
+          sbdo_det_from_ab(bs->a + ja * bs->W, bs->b + jb * bs->W, L, detj);
           acc += sbdo_hij(deti, detj, in) * Wk[ja * nb + jb];
         }
       }
@@ -493,12 +499,12 @@ typedef struct {
 } rdm_acc;
 
 /* correlation.h:15-44 */
-static void zero_diff_corr(const uint64_t det[2], double wi, rdm_acc *r) {
+static void zero_diff_corr(const uint64_t det[SBDO_DW], double wi, rdm_acc *r) {
   const int L = r->L, n2 = 2 * L;
   const int64_t L2 = (int64_t)L * L, L3 = L2 * L, L4 = L3 * L;
-  int closed[128];
+  int closed[256];
   int nc = 0;
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < SBDO_DW; ++w) {
     uint64_t bits = det[w];
     while (bits) {
       int pos = __builtin_ctzll(bits);
@@ -524,14 +530,14 @@ static void zero_diff_corr(const uint64_t det[2], double wi, rdm_acc *r) {
 }
 
 /* correlation.h:49-86: i = bra-only spin-orbital, a = ket-only */
-static void one_diff_corr(const uint64_t det[2], double wi, double wj, int i, int a,
+static void one_diff_corr(const uint64_t det[SBDO_DW], double wi, double wj, int i, int a,
                           rdm_acc *r) {
   const int L = r->L;
   const int64_t L2 = (int64_t)L * L, L3 = L2 * L, L4 = L3 * L;
   double sgn = (double)sbdo_parity(det, i < a ? i : a, i < a ? a : i);
   int oi = i / 2, si = i % 2, oa = a / 2;
   r->one[si * L2 + oi + L * oa] += wi * wj * sgn;
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < SBDO_DW; ++w) {
     uint64_t bits = det[w];
     while (bits) {
       int pos = __builtin_ctzll(bits);
@@ -549,7 +555,7 @@ static void one_diff_corr(const uint64_t det[2], double wi, double wj, int i, in
 }
 
 /* correlation.h:91-130 */
-static void two_diff_corr(const uint64_t det[2], double wi, double wj, int i, int j, int a,
+static void two_diff_corr(const uint64_t det[SBDO_DW], double wi, double wj, int i, int j, int a,
                           int b, rdm_acc *r) {
   const int L = r->L;
   const int64_t L2 = (int64_t)L * L, L4 = L2 * L2;
@@ -573,11 +579,11 @@ static void two_diff_corr(const uint64_t det[2], double wi, double wj, int i, in
 }
 
 /* correlation.h:135-189: DetI = bra, DetJ = ket */
-static void corr_term(const uint64_t deti[2], const uint64_t detj[2], double wi, double wj,
+static void corr_term(const uint64_t deti[SBDO_DW], const uint64_t detj[SBDO_DW], double wi, double wj,
                       rdm_acc *r) {
   int c[4], d[4];
   int nc = 0, nd = 0;
-  for (int w = 0; w < 2; ++w) {
+  for (int w = 0; w < SBDO_DW; ++w) {
     uint64_t dc = deti[w] & ~detj[w], dd = detj[w] & ~deti[w];
     while (dc) {
       if (nc < 4) c[nc] = w * 64 + __builtin_ctzll(dc);
@@ -606,8 +612,8 @@ void sbdo_correlation(const sbdo_basis *bs, const double *W, double *onebody,
   rdm_acc main_acc = {L, onebody, twobody};
   for (int64_t ia = 0; ia < na; ++ia)
     for (int64_t ib = 0; ib < nb; ++ib) {
-      uint64_t det[2];
-      sbdo_det_from_ab(bs->a[ia], bs->b[ib], L, det);
+      uint64_t det[SBDO_DW];
+      sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + ib * bs->W, L, det);
       zero_diff_corr(det, W[ia * nb + ib], &main_acc);
     }
   const sbdo_conn *A = &bs->ca, *Bc = &bs->cb;
@@ -619,35 +625,35 @@ void sbdo_correlation(const sbdo_basis *bs, const double *W, double *onebody,
     loc.two = (double *)calloc((size_t)(4 * L4), sizeof(double));
 #pragma omp for schedule(dynamic, 1)
     for (int64_t ia = 0; ia < na; ++ia) {
-      uint64_t deti[2], detj[2];
+      uint64_t deti[SBDO_DW], detj[SBDO_DW];
       for (int64_t ib = 0; ib < nb; ++ib) {
         double wi = W[ia * nb + ib];
-        sbdo_det_from_ab(bs->a[ia], bs->b[ib], L, deti);
+        sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + ib * bs->W, L, deti);
         for (int64_t k = A->off_s[ia]; k < A->off_s[ia + 1]; ++k) {
           int64_t ja = A->idx_s[k];
-          sbdo_det_from_ab(bs->a[ja], bs->b[ib], L, detj);
+          sbdo_det_from_ab(bs->a + ja * bs->W, bs->b + ib * bs->W, L, detj);
           corr_term(deti, detj, wi, W[ja * nb + ib], &loc);
         }
         for (int64_t k = A->off_d[ia]; k < A->off_d[ia + 1]; ++k) {
           int64_t ja = A->idx_d[k];
-          sbdo_det_from_ab(bs->a[ja], bs->b[ib], L, detj);
+          sbdo_det_from_ab(bs->a + ja * bs->W, bs->b + ib * bs->W, L, detj);
           corr_term(deti, detj, wi, W[ja * nb + ib], &loc);
         }
         for (int64_t k = Bc->off_s[ib]; k < Bc->off_s[ib + 1]; ++k) {
           int64_t jb = Bc->idx_s[k];
-          sbdo_det_from_ab(bs->a[ia], bs->b[jb], L, detj);
+          sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + jb * bs->W, L, detj);
           corr_term(deti, detj, wi, W[ia * nb + jb], &loc);
         }
         for (int64_t k = Bc->off_d[ib]; k < Bc->off_d[ib + 1]; ++k) {
           int64_t jb = Bc->idx_d[k];
-          sbdo_det_from_ab(bs->a[ia], bs->b[jb], L, detj);
+          sbdo_det_from_ab(bs->a + ia * bs->W, bs->b + jb * bs->W, L, detj);
           corr_term(deti, detj, wi, W[ia * nb + jb], &loc);
         }
         for (int64_t k = A->off_s[ia]; k < A->off_s[ia + 1]; ++k) {
           int64_t ja = A->idx_s[k];
           for (int64_t m = Bc->off_s[ib]; m < Bc->off_s[ib + 1]; ++m) {
             int64_t jb = Bc->idx_s[m];
-            sbdo_det_from_ab(bs->a[ja], bs->b[jb], L, detj);
+            sbdo_det_from_ab(bs->a + ja * bs->W, bs->b + jb * bs->W, L, detj);
             corr_term(deti, detj, wi, W[ja * nb + jb], &loc);
           }
         }
@@ -672,8 +678,8 @@ void sbdo_occupation(const sbdo_basis *bs, const double *W, double *density) {
     for (int64_t ib = 0; ib < nb; ++ib) {
       double w2 = W[ia * nb + ib] * W[ia * nb + ib];
       for (int io = 0; io < L; ++io) {
-        density[2 * io] += w2 * (double)((bs->a[ia] >> io) & 1u);
-        density[2 * io + 1] += w2 * (double)((bs->b[ib] >> io) & 1u);
+        density[2 * io] += w2 * (double)((bs->a[ia * bs->W + (io >> 6)] >> (io & 63)) & 1u);
+        density[2 * io + 1] += w2 * (double)((bs->b[ib * bs->W + (io >> 6)] >> (io & 63)) & 1u);
       }
     }
 }

@@ -6,9 +6,10 @@
  * "port" CPU baseline in the benchmark.  Nothing in the product links, loads or calls it.
  *
  * Storage width: the reference packs `bit_length` (default 20) bits per size_t word.  The
- * oracle packs 64 bits per word (strings: one uint64, L <= 64; interleaved determinants: two
- * uint64).  Every function below depends only on bit positions, never on the word width, so
- * the semantics are the reference's.
+ * oracle packs 64 bits per word (strings: W = 1 uint64 for L <= 64, W = 2 for L <= 128, least
+ * significant word first; interleaved determinants: SBDO_DW = 4 uint64).  Every function below
+ * depends only on bit positions, never on the word width, so the semantics are the
+ * reference's.
  *
  * Pinning (how each function is checked, see tests/test_oracle.py):
  *   setup_integrals  -- compiled reference fixtures; 8-fold symmetry identities
@@ -29,8 +30,10 @@
 extern "C" {
 #endif
 
+#define SBDO_DW 4      /* uint64 words of an interleaved determinant (2L <= 256 bits) */
+
 typedef struct {
-  int L;               /* spatial orbitals (<= 64) */
+  int L;               /* spatial orbitals (<= 128) */
   double I0;           /* core energy */
   const double *I1;    /* (2L)x(2L) spin-orbital one-electron matrix, I1[i*2L+j] */
   const double *I2;    /* packed spatial (pq|rs), reference twoInt::store layout */
@@ -47,8 +50,9 @@ typedef struct {
 typedef struct {
   int L;
   int64_t na, nb;
-  const uint64_t *a, *b;
+  const uint64_t *a, *b;   /* na (nb) strings of W words each */
   sbdo_conn ca, cb;
+  int W;
 } sbdo_basis;
 
 /* chemistry/basic/makeintegrals.h:10-55 (caller sizes: I1 4L^2, I2 B(B+1)/2 with
@@ -59,20 +63,21 @@ int sbdo_setup_integrals(int L, int64_t nent, const double *val, const int32_t *
 /* chemistry/basic/integrals.h:51-59 */
 double sbdo_value(const sbdo_ints *in, int i, int j, int k, int l);
 
-/* chemistry/basic/determinants.h:98-119 */
-void sbdo_det_from_ab(uint64_t a, uint64_t b, int L, uint64_t det[2]);
+/* chemistry/basic/determinants.h:98-119; a, b: one string each (ceil(L/64) words) */
+void sbdo_det_from_ab(const uint64_t *a, const uint64_t *b, int L, uint64_t det[SBDO_DW]);
 
 /* chemistry/basic/determinants.h:205-256: returns the +-1 factor */
-int sbdo_parity(const uint64_t det[2], int start, int end);
+int sbdo_parity(const uint64_t det[SBDO_DW], int start, int end);
 
 /* chemistry/basic/determinants.h:374-398, :505-564 */
-double sbdo_zero_excite(const uint64_t det[2], const sbdo_ints *in);
-double sbdo_hij(const uint64_t bra[2], const uint64_t ket[2], const sbdo_ints *in);
+double sbdo_zero_excite(const uint64_t det[SBDO_DW], const sbdo_ints *in);
+double sbdo_hij(const uint64_t bra[SBDO_DW], const uint64_t ket[SBDO_DW], const sbdo_ints *in);
 
-/* chemistry/tpb/helper.h:132-182 (GenerateExcitation).  Pass 1 counts, pass 2 fills. */
-void sbdo_count_excitations(const uint64_t *s, int64_t n, int L, int64_t *nsingle,
+/* chemistry/tpb/helper.h:132-182 (GenerateExcitation).  Pass 1 counts, pass 2 fills.
+ * s: n strings of W words each. */
+void sbdo_count_excitations(const uint64_t *s, int64_t n, int W, int L, int64_t *nsingle,
                             int64_t *ndouble);
-void sbdo_fill_excitations(const uint64_t *s, int64_t n, int L, const int64_t *off_s,
+void sbdo_fill_excitations(const uint64_t *s, int64_t n, int W, int L, const int64_t *off_s,
                            int64_t *idx_s, const int64_t *off_d, int64_t *idx_d);
 
 /* chemistry/tpb/qcham.h:314-366 */

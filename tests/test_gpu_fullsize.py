@@ -48,15 +48,15 @@ def test_column_matches_slater_condon(c2):
         da, db = popdist(inst.alpha, inst.alpha[ka]), popdist(inst.beta, inst.beta[kb])
         ra = np.flatnonzero(da <= 4)
         rb = np.flatnonzero(db <= 4)
-        ket = np.zeros(2, np.uint64)
-        lib.sbdo_det_from_ab(int(inst.alpha[ka]), int(inst.beta[kb]), L, ket)
+        ket = np.zeros(O.DW, np.uint64)
+        lib.sbdo_det_from_ab(inst.alpha[ka:ka + 1], inst.beta[kb:kb + 1], L, ket)
         expect = {}
         for ia in ra:
             for ib in rb:
                 if da[ia] + db[ib] > 4:
                     continue
-                bra = np.zeros(2, np.uint64)
-                lib.sbdo_det_from_ab(int(inst.alpha[ia]), int(inst.beta[ib]), L, bra)
+                bra = np.zeros(O.DW, np.uint64)
+                lib.sbdo_det_from_ab(inst.alpha[ia:ia + 1], inst.beta[ib:ib + 1], L, bra)
                 expect[ia * nb + ib] = lib.sbdo_hij(bra, ket, ints)
         nz = np.flatnonzero(col)
         assert set(nz.tolist()) <= set(expect.keys())

@@ -22,7 +22,9 @@ from sbd_amd import synth
 import bruteforce as BF
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["l6_2a2b", "l8_3a2b", "c2fam_120x110", "c2fam_300x280", "c3fam_200x220"]
+# w70_3a3b: 70 orbitals, every 3-electron string of orbitals 59..68 per spin (two 64-bit words
+# per string here, four 20-bit words in the reference)
+CASES = ["l6_2a2b", "l8_3a2b", "c2fam_120x110", "c2fam_300x280", "c3fam_200x220", "w70_3a3b"]
 
 
 def load(name):
@@ -37,9 +39,14 @@ _systems = {}
 def system(name):
     if name not in _systems:
         meta, z = load(name)
-        inst = synth.make_custom(meta["norb"], meta["n_alpha_el"], meta["n_beta_el"], meta["rank"],
-                                 meta["n_alpha"], meta["n_beta"], seed_ints=meta["seed_ints"],
-                                 seed_alpha=meta["seed_alpha"], seed_beta=meta["seed_beta"])
+        if meta["recipe"] == "sbd_amd.synth.make_window":
+            lo, hi = meta["window"]
+            inst = synth.make_window(meta["norb"], lo, hi, meta["n_alpha_el"], meta["n_beta_el"],
+                                     meta["rank"], seed_ints=meta["seed_ints"])
+        else:
+            inst = synth.make_custom(meta["norb"], meta["n_alpha_el"], meta["n_beta_el"], meta["rank"],
+                                     meta["n_alpha"], meta["n_beta"], seed_ints=meta["seed_ints"],
+                                     seed_alpha=meta["seed_alpha"], seed_beta=meta["seed_beta"])
         _systems[name] = (meta, z, inst, O.System.from_instance(inst))
     return _systems[name]
 
@@ -108,10 +115,10 @@ def test_davidson_energy_and_rdm(name):
     assert np.max(np.abs(so.occupation(r["W"]) - z["density"])) < 1e-6
 
 
-@pytest.mark.parametrize("name", ["c2fam_300x280", "c3fam_200x220"])
+@pytest.mark.parametrize("name", ["c2fam_300x280", "c3fam_200x220", "w70_3a3b"])
 def test_rdm_of_seeded_vector_matches_reference(name):
     """Correlation and OccupationDensity of the same seeded vector as the compiled reference:
-    36 orbitals (two 20-bit words per string) and 48 (three)."""
+    36 orbitals (two 20-bit words per string), 48 (three) and 70 (four)."""
     meta, z, inst, so = system(name)
     W = np.random.default_rng(meta["rdmvec_seed"]).standard_normal(so.ndet)
     W /= np.linalg.norm(W)
@@ -124,6 +131,15 @@ def test_rdm_of_seeded_vector_matches_reference(name):
     assert abs(two @ np.cos(0.1 * np.arange(two.size)) - float(z["rdmvec_two_wsum"][0])) < 1e-14 * scale
     # the reference's occupation sums run over OpenMP threads: same terms, another order
     assert np.max(np.abs(so.occupation(W) - z["rdmvec_density"])) < 1e-12
+
+
+def test_davidson_trace_two_word_strings():
+    """The 70-orbital fixture's first Davidson steps (its full run, 499 steps, takes minutes
+    here; its energy is pinned on the GPU side, test_gpu_golden.py)."""
+    meta, z, inst, so = system("w70_3a3b")
+    r = so.davidson(max_it=1, num_block=10, eps=1e-8)
+    assert r["steps"] == 10
+    assert np.allclose(r["trace"][:3], z["trace"][:3], rtol=0, atol=1e-9)
 
 
 def test_c1_fixture_consistency():
