@@ -10,7 +10,10 @@ shows), L2 hit rate, TA busy fraction, L1 pending-stall fraction, instruction co
 clock.  A phase with several launches per build (the row passes' ket blocks) is summed over
 them.
 
-    python -m sbd_amd.pmc_summary gpurun_out/pmcfp64 c2/fp64 [profiles/counters.json]
+    python -m sbd_amd.pmc_summary <prefix> c2/fp64 [profiles/counters.json]
+
+where <prefix>_trace, <prefix>_fetch, ... are the rocprofv3 output folders of one workload and
+precision (their -d arguments).
 """
 import collections
 import csv
