@@ -7,6 +7,9 @@
 //   shim_io configs <dets .bin file> <norb> <bit_length> <out prefix>
 //       sbd::LoadFromAlphaDets: <out>.config.u64 (the configurations' words, flat) and the
 //       word count per configuration in the printed line
+//   shim_io pack <words file> <norb> <bit_length> <out prefix>
+//       the header's packing of bit_length-bit strings (the file: ceil(norb / bit_length)
+//       uint64 words per string, flat) into the engine's 64-bit words: <out>.packed.u64
 #include <cstdio>
 #include <fstream>
 #include <string>
@@ -26,6 +29,25 @@ int main(int argc, char** argv) {
                 config.empty() ? size_t(0) : config[0].size());
     MPI_Finalize();
     return 0;
+  }
+  if (argc == 6 && std::string(argv[1]) == "pack") {
+    const size_t norb = std::stoul(argv[3]), bl = std::stoul(argv[4]), nw = (norb + bl - 1) / bl;
+    std::ifstream in(argv[2], std::ios::binary);
+    std::vector<std::vector<size_t>> d;
+    std::vector<size_t> w(nw);
+    while (in.read(reinterpret_cast<char*>(w.data()), sizeof(size_t) * nw)) d.push_back(w);
+    int rc = 0;
+    try {
+      const std::vector<uint64_t> p = sbd::amd_detail::pack(d, bl, norb);
+      std::ofstream f(std::string(argv[5]) + ".packed.u64", std::ios::binary);
+      f.write(reinterpret_cast<const char*>(p.data()), sizeof(uint64_t) * p.size());
+      std::printf("{\"op\":\"pack\",\"n\":%zu,\"words\":%d}\n", d.size(), sbd::amd_detail::words_for(norb));
+    } catch (const std::invalid_argument& e) {
+      std::printf("{\"op\":\"pack\",\"error\":\"%s\"}\n", e.what());
+      rc = 1;
+    }
+    MPI_Finalize();
+    return rc;
   }
   if (argc < 6) {
     std::fprintf(stderr, "usage: shim_io <adets> <bdets> <norb> <bit_length> <out prefix> [vector] | configs <bin> <norb> <bit_length> <out>\n");

@@ -4,6 +4,7 @@ SaveMatrixFormWF (restart.h:662-780).  Host code only.  The expected words are c
 from the strings themselves (a string read right to left is orbital 0 upwards; a .bin record
 is that number in big-endian bytes); where the compiled reference driver is built
 (oracle/_ref/sbd_ref), its files for the same inputs must be identical byte for byte."""
+import json
 import os
 import shutil
 import subprocess
@@ -172,3 +173,31 @@ def test_load_from_alpha_dets_configurations(shim_io, tmp_path, norb, bl):
     got = np.fromfile(out + ".config.u64", dtype=np.uint64)
     assert got.size == n * n * nw
     assert np.array_equal(got, np.array(want, dtype=np.uint64))
+
+
+@pytest.mark.parametrize("norb,bl", [(40, 20), (70, 20), (128, 20), (100, 64), (70, 13)])
+def test_header_packs_strings_into_64_bit_words(shim_io, tmp_path, norb, bl):
+    """The drop-in header's pack (the semantics of change_bitlength, bit_manipulation.h:863-893):
+    bit_length-bit strings of norb orbitals become one 64-bit word each up to 64 orbitals and
+    two (low word first) up to 128; more are refused."""
+    rng = np.random.default_rng(norb + bl)
+    masks = [sum(1 << int(p) for p in rng.choice(norb, int(rng.integers(1, min(norb, 20))), replace=False))
+             for _ in range(50)]
+    nw = (norb + bl - 1) // bl
+    words = np.array([(m >> (bl * w)) & ((1 << bl) - 1) for m in masks for w in range(nw)], np.uint64)
+    words.tofile(tmp_path / "s.u64")
+    out = str(tmp_path / "p")
+    r = subprocess.run([shim_io, "pack", str(tmp_path / "s.u64"), str(norb), str(bl), out],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr[-2000:]
+    W = 1 if norb <= 64 else 2
+    assert json.loads(r.stdout.strip().splitlines()[-1])["words"] == W
+    want = [(m >> (64 * k)) & 0xFFFFFFFFFFFFFFFF for m in masks for k in range(W)]
+    assert np.array_equal(np.fromfile(out + ".packed.u64", np.uint64), np.array(want, np.uint64))
+
+
+def test_header_refuses_more_than_128_orbitals(shim_io, tmp_path):
+    np.zeros(7, np.uint64).tofile(tmp_path / "s.u64")  # one 129-orbital string of 20-bit words
+    r = subprocess.run([shim_io, "pack", str(tmp_path / "s.u64"), "129", "20", str(tmp_path / "p")],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "at most 128 orbitals" in r.stdout

@@ -84,3 +84,37 @@ def test_bench_solver_variants_are_valid_options():
         assert opts.get("fp32_acc", 0) in (0, 1), name
         if "mlp" in opts:
             assert os.path.exists(os.path.join(ROOT, opts["mlp"])), name
+
+
+def test_words_round_trip_and_window_strings():
+    """as_words / masks_of invert each other; a window's strings are exactly the full space of
+    the window's orbitals, two words each above 64 orbitals."""
+    rng = np.random.default_rng(0)
+    masks = [int(rng.integers(1, 2**62)) | (int(rng.integers(0, 2**62)) << 64) for _ in range(20)]
+    assert synth.masks_of(synth.as_words(masks, 128)) == masks
+    w = synth.make_window(70, 59, 69, 3, 3, 8, seed_ints=6)
+    got = synth.masks_of(w.alpha)
+    assert w.alpha.shape == (120, 2) and got == sorted(got)
+    assert all(bin(m).count("1") == 3 and m >> 69 == 0 and m & ((1 << 59) - 1) == 0 for m in got)
+    assert len(set(got)) == 120  # C(10, 3)
+
+
+def test_embed_renames_orbitals():
+    """embed: strings shifted by the renaming, the integrals of the renamed block copied."""
+    base = synth.make_custom(10, 3, 3, 8, 60, 50, seed_ints=3)
+    big = synth.embed(base, 70, 58)
+    assert synth.masks_of(big.alpha) == [m << 58 for m in synth.masks_of(base.alpha)]
+    assert synth.masks_of(big.beta) == [m << 58 for m in synth.masks_of(base.beta)]
+    blk = slice(58, 68)
+    assert np.array_equal(big.ints.h1[blk, blk], base.ints.h1)
+    assert np.array_equal(big.ints.eri[blk, blk, blk, blk], base.ints.eri)
+    assert big.ndet == base.ndet
+
+
+def test_one_word_sampling_is_unchanged():
+    """The sampled strings behind the committed fixtures keep their one-word form and values:
+    the first three alpha strings of the 36-orbital 120 x 110 case (tests/golden/
+    c2fam_120x110.npz was written from this sample)."""
+    inst = synth.make_custom(36, 8, 8, 72, 120, 110, seed_ints=0)
+    assert inst.alpha.ndim == 1 and inst.alpha.dtype == np.uint64
+    assert int(inst.alpha[0]) == 255 and int(inst.alpha[1]) == 383 and int(inst.alpha[2]) == 447
