@@ -394,7 +394,8 @@ def main():
                     help="mixed-kernel variant of the timed one-vector build (sbdx_set_mix_variant; 0: default)")
     ap.add_argument("--rename", type=int, nargs=2, default=None, metavar=("NORB", "SHIFT"),
                     help="the same problem with orbital p renamed p + SHIFT among NORB orbitals "
-                         "(NORB > 64: two-word strings; same H, same sigma)")
+                         "(NORB > 64: two-word strings; same H, same sigma); times the sigma "
+                         "build only, unless solves are named with --solve")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -577,8 +578,9 @@ def main():
             torch.cuda.empty_cache()
         x = y = None
 
+    # (a renamed workload is a sigma-build measurement: solves only when named with --solve)
     default_solve = (args.solve is None and not args.no_solve and args.workload == "c2"
-                     and args.strings is None and args.prec == "fp64")
+                     and args.strings is None and args.prec == "fp64" and not args.rename)
     if args.solve is not None or default_solve:
         names = args.solve if args.solve is not None else list(DEFAULT_SOLVES)
         names = names or ["plain", "mixed_block2"]
@@ -654,7 +656,7 @@ def main():
     if args.batch:
         out["batch"] = batch_bench(args.batch, args.batch_prec)
 
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not args.rename:
         fam = args.workload if args.workload in ("c2", "c3") else "c2"
         out["cpu_baseline"] = cpu_baseline(fam, nnz)
 
